@@ -1,0 +1,103 @@
+/* indialign_amd — MI355X (gfx950) TM-score superposition search.
+ *
+ * Drop-in replacement for the reference library's C ABI.  The struct layouts
+ * and the four reference entry points are identical to
+ * indialign_c/api.h:10-46, so existing ctypes callers (tests/test_cpp_smoke.py,
+ * benchmark/fair_benchmark.py:21-46, benchmark/pdb_*.py) load
+ * libindialign.so from this repository unchanged.
+ *
+ * All pointers are borrowed, read-only, caller-owned and must stay valid for
+ * the duration of the call.  Coordinates are interleaved AoS double[3N];
+ * masks are uint8 with values 0 or 1.  The rotation is applied as
+ * out_d = sum_k P_k * rotation[3k+d] + translation[d].
+ */
+#ifndef INDIALIGN_AMD_H
+#define INDIALIGN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* api.h:10-20.  backend_mode and max_mem_gb are accepted and ignored (the
+ * reference ignores them too, indialign.cpp:290-297); request_cuda is
+ * ignored: the search always runs on the GPU. */
+typedef struct {
+    int backend_mode;
+    int use_fragment_search;
+    int max_iter;
+    int dp_iter;
+    double max_mem_gb;
+    int request_cuda;
+    double rescue_score_1;
+    double rescue_score_2;
+    double early_term_score;
+} NativeConfig;
+
+/* api.h:22-30 */
+typedef struct {
+    int length;
+    const double *pred_xyz;
+    const double *native_xyz;
+    const uint8_t *valid_mask;
+    const uint8_t *pred_valid_mask;
+    const uint8_t *native_valid_mask;
+    double lnorm;
+} NativePairInput;
+
+/* api.h:32-36 */
+typedef struct {
+    double score;
+    double rotation[9];
+    double translation[3];
+} NativePairResult;
+
+/* Replaces indialign_default_config (api.h:38, indialign.cpp:272-282):
+ * same defaults. */
+void indialign_default_config(NativeConfig *cfg);
+
+/* Replaces indialign_tmscore_search (api.h:39-41, indialign.cpp:284-303).
+ * Returns -1 when length <= 0 (as the reference), -2 when no usable GPU is
+ * present or a device error occurred (message on stderr; the reference
+ * never had this case because it fell back to its CPU path), else 0.
+ * The main seed bank follows the reference's multithreaded single-pair
+ * semantics (no early termination, search.h:100-136). */
+int indialign_tmscore_search(const NativePairInput *inp, const NativeConfig *cfg,
+                             NativePairResult *out);
+
+/* Replaces indialign_tmscore_search_batch (api.h:42-45,
+ * indialign.cpp:305-313).  All pairs run together in batched launches; the
+ * main seed bank follows the reference's batch (serial) semantics, i.e.
+ * early_term_score applies (search.h:83).  Returns 0, or -2 on a device
+ * error.  Pairs with length <= 0 are left untouched (the reference ignored
+ * per-pair return codes). */
+int indialign_tmscore_search_batch(int n_pairs, const NativePairInput *inputs,
+                                   const NativeConfig *cfg, NativePairResult *outputs);
+
+/* Replaces indialign_cuda_available (api.h:46, indialign.cpp:315-321):
+ * 1 when a HIP device is present, else 0 (the name is kept for the ABI). */
+int indialign_cuda_available(void);
+
+/* ---- extensions (no counterpart in the reference) ---------------------- */
+
+/* Batch search with an explicit seed-bank convention: serial_bank = 1 applies
+ * early_term_score to the main seed bank (the reference's batch / one-thread
+ * semantics), 0 never stops early (its multithreaded single-pair
+ * semantics).  Same return codes as the batch entry. */
+int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
+                        NativePairResult *outputs, int serial_bank);
+
+/* Work and time of the last call on this thread's engine.  Fills up to n
+ * doubles:
+ *   [0] wall ms  [1] candidates refined  [2] NW cells filled
+ *   [3] chains run  [4] kernel launches
+ *   [5] chain-kernel ms  [6] bank-kernel ms  (device time, HIP events)
+ * Returns the number of values written. */
+int indialign_last_stats(double *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,176 @@
+"""indialign_amd: MI355X (gfx950) TM-score superposition search.
+
+The product is the shared library ``libindialign.so`` in this package; it
+exports the reference C ABI (indialign_c/api.h:38-46) unchanged, so callers
+written against the reference (ctypes, README.md:17-65) load it as a drop-in.
+This module is the Python mirror of that interface: the same three ctypes
+structures (api.h:10-36) and thin wrappers that pack numpy arrays.
+
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libindialign.so")
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_u8_p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class NativeConfig(ctypes.Structure):
+    """api.h:10-20"""
+    _fields_ = [("backend_mode", ctypes.c_int), ("use_fragment_search", ctypes.c_int),
+                ("max_iter", ctypes.c_int), ("dp_iter", ctypes.c_int),
+                ("max_mem_gb", ctypes.c_double), ("request_cuda", ctypes.c_int),
+                ("rescue_score_1", ctypes.c_double), ("rescue_score_2", ctypes.c_double),
+                ("early_term_score", ctypes.c_double)]
+
+
+class NativePairInput(ctypes.Structure):
+    """api.h:22-30"""
+    _fields_ = [("length", ctypes.c_int), ("pred_xyz", c_double_p), ("native_xyz", c_double_p),
+                ("valid_mask", c_u8_p), ("pred_valid_mask", c_u8_p),
+                ("native_valid_mask", c_u8_p), ("lnorm", ctypes.c_double)]
+
+
+class NativePairResult(ctypes.Structure):
+    """api.h:32-36"""
+    _fields_ = [("score", ctypes.c_double), ("rotation", ctypes.c_double * 9),
+                ("translation", ctypes.c_double * 3)]
+
+
+class IndialignError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load libindialign.so; raises if it was not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise IndialignError(f"{p} not found: build it with `python -m indialign_amd.build`")
+    lib = ctypes.CDLL(p)
+    lib.indialign_default_config.argtypes = [ctypes.POINTER(NativeConfig)]
+    lib.indialign_default_config.restype = None
+    lib.indialign_tmscore_search.argtypes = [ctypes.POINTER(NativePairInput),
+                                             ctypes.POINTER(NativeConfig),
+                                             ctypes.POINTER(NativePairResult)]
+    lib.indialign_tmscore_search.restype = ctypes.c_int
+    lib.indialign_tmscore_search_batch.argtypes = [ctypes.c_int, ctypes.POINTER(NativePairInput),
+                                                   ctypes.POINTER(NativeConfig),
+                                                   ctypes.POINTER(NativePairResult)]
+    lib.indialign_tmscore_search_batch.restype = ctypes.c_int
+    lib.indialign_search_ex.argtypes = [ctypes.c_int, ctypes.POINTER(NativePairInput),
+                                        ctypes.POINTER(NativeConfig),
+                                        ctypes.POINTER(NativePairResult), ctypes.c_int]
+    lib.indialign_search_ex.restype = ctypes.c_int
+    lib.indialign_cuda_available.argtypes = []
+    lib.indialign_cuda_available.restype = ctypes.c_int
+    lib.indialign_last_stats.argtypes = [c_double_p, ctypes.c_int]
+    lib.indialign_last_stats.restype = ctypes.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def default_config(**overrides) -> NativeConfig:
+    cfg = NativeConfig()
+    load_library().indialign_default_config(ctypes.byref(cfg))
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def gpu_available() -> bool:
+    return bool(load_library().indialign_cuda_available())
+
+
+class _Packed:
+    """Owns contiguous copies of the pair arrays behind NativePairInput structs."""
+
+    def __init__(self, pairs):
+        self.keep = []
+        self.inps = (NativePairInput * max(1, len(pairs)))()
+        for k, p in enumerate(pairs):
+            pred, native, valid, pv, nv, lnorm = p
+            pf = np.ascontiguousarray(np.asarray(pred, np.float64).reshape(-1))
+            nf = np.ascontiguousarray(np.asarray(native, np.float64).reshape(-1))
+            n = pf.size // 3
+            v = np.ascontiguousarray(valid if valid is not None else np.ones(n), dtype=np.uint8)
+            pvm = np.ascontiguousarray(pv if pv is not None else v, dtype=np.uint8)
+            nvm = np.ascontiguousarray(nv if nv is not None else v, dtype=np.uint8)
+            self.keep += [pf, nf, v, pvm, nvm]
+            self.inps[k] = NativePairInput(
+                n, pf.ctypes.data_as(c_double_p), nf.ctypes.data_as(c_double_p),
+                v.ctypes.data_as(c_u8_p), pvm.ctypes.data_as(c_u8_p), nvm.ctypes.data_as(c_u8_p),
+                float(lnorm if lnorm is not None else n))
+        self.n = len(pairs)
+
+
+def _as_tuple(p):
+    if isinstance(p, tuple):
+        return p + (None,) * (6 - len(p))
+    return (p.pred, p.native, p.valid, p.pvalid, p.nvalid, p.lnorm)
+
+
+def _results(res, n):
+    score = np.array([res[k].score for k in range(n)])
+    R = np.array([list(res[k].rotation) for k in range(n)]).reshape(n, 3, 3)
+    t = np.array([list(res[k].translation) for k in range(n)])
+    return score, R, t
+
+
+def tmscore_search(pred, native, valid=None, pred_valid=None, native_valid=None, lnorm=None,
+                   cfg: NativeConfig | None = None):
+    """One pair through indialign_tmscore_search (indialign.cpp:284-303).
+
+    Returns (score, R, t) with R as stored by the C ABI (out_d = sum_k P_k R[k,d] + t_d),
+    i.e. ``aligned = pred @ R + t``; the reference's callers use R.T with pred @ R.T.
+    """
+    lib = load_library()
+    cfg = cfg or default_config()
+    pk = _Packed([(pred, native, valid, pred_valid, native_valid, lnorm)])
+    res = NativePairResult()
+    rc = lib.indialign_tmscore_search(ctypes.byref(pk.inps[0]), ctypes.byref(cfg),
+                                      ctypes.byref(res))
+    if rc != 0:
+        raise IndialignError(f"indialign_tmscore_search returned {rc}")
+    s, R, t = _results([res], 1)
+    return float(s[0]), R[0], t[0]
+
+
+def tmscore_search_batch(pairs, cfg: NativeConfig | None = None, serial_bank: int | None = None):
+    """Many pairs in one batched call.  `pairs` holds synth.Pair objects or
+    (pred, native, valid, pred_valid, native_valid, lnorm) tuples.  Returns
+    (scores[n], R[n,3,3], t[n,3]).  serial_bank None uses the batch entry's
+    convention (early termination applies); 0/1 selects it explicitly."""
+    lib = load_library()
+    cfg = cfg or default_config()
+    pk = _Packed([_as_tuple(p) for p in pairs])
+    res = (NativePairResult * max(1, pk.n))()
+    if serial_bank is None:
+        rc = lib.indialign_tmscore_search_batch(pk.n, pk.inps, ctypes.byref(cfg), res)
+    else:
+        rc = lib.indialign_search_ex(pk.n, pk.inps, ctypes.byref(cfg), res, int(serial_bank))
+    if rc != 0:
+        raise IndialignError(f"batch search returned {rc}")
+    return _results(res, pk.n)
+
+
+def last_stats() -> dict:
+    """Work counters of the last call (see indialign_last_stats in the header)."""
+    buf = (ctypes.c_double * 8)()
+    load_library().indialign_last_stats(buf, 8)
+    keys = ["wall_ms", "candidates", "nw_cells", "chains", "launches", "chain_ms", "bank_ms"]
+    return {k: buf[i] for i, k in enumerate(keys)}

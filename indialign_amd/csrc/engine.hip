@@ -1,0 +1,1179 @@
+// Host engine and C ABI of the gfx950 TM-score superposition search.
+//
+// The engine runs the stages of the reference pipeline (indialign.cpp:29-266)
+// in the same order and with the same merge rules, for a whole batch of pairs
+// at a time.  Every stage is a batched device launch over all pairs that
+// take part in it; the host only enumerates work (seed windows, rescue work
+// items, offsets), applies the stage gates and performs the strict-'>' merges
+// on per-pair results copied back after each stage.
+#include "ia_kernels.h"
+#include "../../include/indialign.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace ia {
+
+size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds);
+int chain_blocks_per_cu(size_t lds);
+
+namespace {
+
+struct DevError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define IA_CHECK(x)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess)                                                         \
+            throw DevError(std::string(#x) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+// Grow-only device buffer.
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    template <class T>
+    T *get(size_t n) {
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        if (bytes > cap) {
+            if (p) IA_CHECK(hipFree(p));
+            p = nullptr;
+            const size_t want = std::max(bytes, cap + cap / 2);
+            IA_CHECK(hipMalloc(&p, want));
+            cap = want;
+        }
+        return (T *)p;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+Hit identity_hit(double s) {
+    Hit h;
+    set_identity(h.R, h.t);
+    h.score = s;
+    return h;
+}
+
+// d0 normalisation on the host with the C library (core.h:12-25).
+void d0_params(double L, double &d0, double &d0s, double &sd8) {
+    const double Lc = std::max(L, 1.0);
+    if (Lc <= 11.0) d0 = 0.3;
+    else if (Lc <= 15.0) d0 = 0.4;
+    else if (Lc <= 19.0) d0 = 0.5;
+    else if (Lc <= 23.0) d0 = 0.6;
+    else d0 = 0.7;
+    if (Lc >= 30.0) d0 = 0.6 * std::sqrt(Lc - 0.5) - 2.5;
+    if (d0 < 0.3) d0 = 0.3;
+    const double v = 1.24 * std::cbrt(std::max(Lc - 15.0, 0.0)) - 1.8;
+    d0s = std::clamp(v, 4.5, 8.0);
+    sd8 = 1.5 * std::pow(Lc, 0.3) + 3.5;
+}
+
+// Rescue schedules (rescue.h:12-46).
+std::vector<int> rescue_lengths(int plen, int nlen, bool dense) {
+    const int ml = std::min(plen, nlen);
+    std::vector<int> out;
+    if (ml < 4) return out;
+    if (dense) {
+        const int c[9] = {ml, std::min(100, ml / 2), std::min(32, ml), std::min(24, ml),
+                          std::min(20, ml / 3), 16, 12, 8, 4};
+        for (int x : c)
+            if (x >= 4 && std::find(out.begin(), out.end(), x) == out.end()) out.push_back(x);
+    } else {
+        const int a = std::min(20, ml / 3), b = std::min(100, ml / 2);
+        if (a >= 4) out.push_back(a);
+        if (b >= 4 && b != a) out.push_back(b);
+    }
+    return out;
+}
+
+int rescue_stride(int len, bool dense) {
+    int j;
+    if (dense) {
+        j = len > 350 ? 50 : len > 300 ? 30 : len > 200 ? 10 : 5;
+        return std::max(1, std::min(j, std::max(1, len / 4)));
+    }
+    j = len > 250 ? 45 : len > 200 ? 35 : len > 150 ? 25 : 15;
+    return std::max(1, std::min(j, std::max(1, len / 3)));
+}
+
+struct HostPair {
+    int src;  // index in the caller's array
+    int N, V, plen, nlen;
+    bool cross;
+    PairInfo info;
+    std::vector<SeedDesc> frag;  // global + window seeds (seeds.h:41-82)
+};
+
+constexpr double kNever = 1e300;  // early termination disabled
+
+}  // namespace
+
+class Engine {
+  public:
+    Engine() {
+        IA_CHECK(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+        int dev = 0;
+        IA_CHECK(hipGetDevice(&dev));
+        IA_CHECK(hipDeviceGetAttribute(&n_cu_, hipDeviceAttributeMultiprocessorCount, dev));
+        IA_CHECK(hipEventCreate(&e0_));
+        IA_CHECK(hipEventCreate(&e1_));
+    }
+    ~Engine() {
+        for (DBuf *b : all_bufs()) b->release();
+        (void)hipEventDestroy(e0_);
+        (void)hipEventDestroy(e1_);
+        (void)hipStreamDestroy(s_);
+    }
+
+    int run(const NativePairInput *inp, int n, const NativeConfig *cfg, bool serial_bank,
+            NativePairResult *out);
+    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+  private:
+    hipStream_t s_;
+    hipEvent_t e0_, e1_;
+    int n_cu_ = 256;
+    const NativeConfig *cfg_ = nullptr;
+    bool serial_ = true;
+    std::vector<HostPair> hp_;
+    std::vector<Hit> best_;
+    std::vector<int> live_;  // pairs still in the pipeline (index into hp_)
+    Arena A_{};
+    int list_cap_ = 1;
+
+    void ensure_lists(int slots, int cap) {
+        list_cap_ = std::max(cap, 1);
+        b_lists_p.get<int>((size_t)slots * list_cap_);
+        b_lists_n.get<int>((size_t)slots * list_cap_);
+        b_list_na.get<int>(slots);
+        b_lout.get<ListOut>(slots);
+    }
+    // Full detailed search (align.h:15-50, use_frag) over a traceback list.
+    BankDesc list_bank(int pair, int slot, int cap, double sd8, long long out_idx) {
+        BankDesc b{};
+        b.pair = pair;
+        b.pts = PTS_LIST;
+        b.n = -1;
+        b.a0 = slot;
+        b.seeds = SEED_IMPLICIT;
+        b.max_starts = 50;
+        b.C = 6;
+        b.iters = 20;
+        b.use_d8 = 1;
+        b.init = INIT_SEED;
+        b.cand_cap = cap;
+        b.out_idx = out_idx;
+        b.et = 1.0;
+        b.sd8 = sd8;
+        static const double mult[6] = {2.0, 1.5, 1.0, 0.75, 0.5, 0.25};
+        for (int c = 0; c < 6; c++) b.cut[c] = hp_[pair].info.d0s * mult[c];
+        return b;
+    }
+    void profile(const char *stage, std::chrono::steady_clock::time_point t0);
+
+    // arena
+    DBuf b_px, b_py, b_pz, b_qx, b_qy, b_qz, b_v, b_pv, b_nv, b_pcx, b_pcy, b_pcz, b_qcx, b_qcy,
+        b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info;
+    // work buffers
+    DBuf b_seeds, b_masks, b_banks, b_blk_bank, b_blk_first, b_cands, b_bank_out, b_lists_p,
+        b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
+        b_red_cnt, b_red_out, b_rs_pair, b_rs_in, b_rs_out, b_pairs, b_aoff, b_acnt, b_ascr,
+        b_soff, b_fofs, b_fcnt, b_moff;
+
+    std::vector<DBuf *> all_bufs() {
+        return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
+                &b_pcz, &b_qcx, &b_qcy, &b_qcz, &b_pmap, &b_nmap, &b_ssp, &b_ssn, &b_info,
+                &b_seeds, &b_masks, &b_banks, &b_blk_bank, &b_blk_first, &b_cands,
+                &b_bank_out, &b_lists_p, &b_lists_n, &b_list_na, &b_lout, &b_groups, &b_xf,
+                &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
+                &b_rs_pair, &b_rs_in, &b_rs_out, &b_pairs, &b_aoff, &b_acnt, &b_ascr, &b_soff,
+                &b_fofs, &b_fcnt, &b_moff};
+    }
+
+    template <class T>
+    T *upload(DBuf &b, const std::vector<T> &v) {
+        T *p = b.get<T>(v.size());
+        if (!v.empty())
+            IA_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s_));
+        return p;
+    }
+    template <class T>
+    std::vector<T> download(const T *p, size_t n) {
+        std::vector<T> v(n);
+        if (n) IA_CHECK(hipMemcpyAsync(v.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, s_));
+        IA_CHECK(hipStreamSynchronize(s_));
+        return v;
+    }
+    void launched() { stats[4] += 1; }
+
+    void prepare(const NativePairInput *inp, int n);
+    void stage_seed_bank();
+    void stage_ss();
+    void stage_gapless();
+    void stage_rescue();
+    void stage_topk();
+    void stage_dp();
+    void stage_weighted();
+
+    // Evaluate banks; returns device results indexed by out_idx (n_out).
+    Hit *run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc> &seeds,
+                   size_t n_out, Hit *out_dev = nullptr);
+    // Launch chain groups (chain_off filled here); returns number of chains.
+    long long run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev, Hit *out_dev);
+    std::vector<Hit> rescore(const std::vector<int> &pairs, const std::vector<Hit> &in);
+    std::vector<Hit> group_reduce(const std::vector<long long> &offs,
+                                 const std::vector<int> &cnts, const Hit *in_dev);
+    // dp_refine (align.h:149-159) for a list of (pair, start); lite selects the
+    // fused one-launch path, non-lite the per-iteration list + bank path.
+    std::vector<Hit> dp_refine(const std::vector<int> &pairs, const std::vector<Hit> &start,
+                               int iters, bool lite);
+    void offer(int a, const Hit &h) {
+        if (h.score > best_[a].score) best_[a] = h;
+    }
+    void cells_nw(int a, long long chains_times_iters) {
+        stats[2] += (double)hp_[a].plen * hp_[a].nlen * (double)chains_times_iters;
+    }
+    std::vector<int> max_seed_table(int nmax, int ms);
+};
+
+void Engine::prepare(const NativePairInput *inp, int n) {
+    hp_.clear();
+    std::vector<double> px, py, pz, qx, qy, qz, pcx, pcy, pcz, qcx, qcy, qcz;
+    std::vector<uint8_t> v, pv, nv;
+    std::vector<int> pmap, nmap;
+    std::vector<PairInfo> info;
+    int off = 0;
+    for (int k = 0; k < n; k++) {
+        const NativePairInput &I = inp[k];
+        HostPair h{};
+        h.src = k;
+        h.N = I.length;
+        if (h.N <= 0 || I.lnorm <= 2.0) continue;  // handled by the caller
+        const int N = h.N;
+        h.info.N = N;
+        h.info.off = off;
+        h.info.Lnorm = I.lnorm;
+        d0_params(I.lnorm, h.info.d0, h.info.d0s, h.info.sd8);
+        // d0 search list (indialign.cpp:54-67)
+        static const double mult[6] = {2.0, 1.5, 1.0, 0.75, 0.5, 0.25};
+        int C = 0;
+        for (double m : mult) h.info.cut[C++] = h.info.d0s * m;
+        for (int a = 1; a <= 5; a++) {
+            const double val = h.info.d0 + a;
+            bool dup = false;
+            for (int e = 0; e < C && !dup; e++) dup = std::fabs(h.info.cut[e] - val) < 0.01;
+            if (!dup) h.info.cut[C++] = val;
+        }
+        h.info.C = C;
+        int V = 0, pl = 0, nl = 0;
+        bool cross = false;
+        std::vector<int> pre(N + 1, 0);
+        for (int i = 0; i < N; i++) {
+            const uint8_t vi = I.valid_mask[i] ? 1 : 0;
+            const uint8_t pi = I.pred_valid_mask[i] ? 1 : 0;
+            const uint8_t ni = I.native_valid_mask[i] ? 1 : 0;
+            V += vi;
+            pre[i + 1] = pre[i] + vi;
+            cross |= pi != ni;
+            px.push_back(I.pred_xyz[3 * i]);
+            py.push_back(I.pred_xyz[3 * i + 1]);
+            pz.push_back(I.pred_xyz[3 * i + 2]);
+            qx.push_back(I.native_xyz[3 * i]);
+            qy.push_back(I.native_xyz[3 * i + 1]);
+            qz.push_back(I.native_xyz[3 * i + 2]);
+            v.push_back(vi);
+            pv.push_back(pi);
+            nv.push_back(ni);
+        }
+        // compact pred-valid / native-valid rows (align.h:66-71)
+        for (int i = 0; i < N; i++)
+            if (pv[off + i]) {
+                pcx.push_back(px[off + i]); pcy.push_back(py[off + i]); pcz.push_back(pz[off + i]);
+                pmap.push_back(i);
+                pl++;
+            }
+        for (int i = pl; i < N; i++) {
+            pcx.push_back(0); pcy.push_back(0); pcz.push_back(0); pmap.push_back(0);
+        }
+        for (int i = 0; i < N; i++)
+            if (nv[off + i]) {
+                qcx.push_back(qx[off + i]); qcy.push_back(qy[off + i]); qcz.push_back(qz[off + i]);
+                nmap.push_back(i);
+                nl++;
+            }
+        for (int i = nl; i < N; i++) {
+            qcx.push_back(0); qcy.push_back(0); qcz.push_back(0); nmap.push_back(0);
+        }
+        h.V = V;
+        h.plen = pl;
+        h.nlen = nl;
+        h.cross = cross;
+        h.info.plen = pl;
+        h.info.nlen = nl;
+        h.info.V = V;
+        // global + window seeds over index positions, >= 3 valid (seeds.h:41-82)
+        if (V >= 3) h.frag.push_back(SeedDesc{0, N});
+        if (cfg_->use_fragment_search && V >= 4) {
+            int lens[9];
+            const int nlens = window_lengths(V, lens);
+            for (int L = 0; L < nlens; L++) {
+                const int cnt = window_start_count(N, lens[L], 50);
+                for (int q = 0; q < cnt; q++) {
+                    const int s0 = window_start(N, lens[L], 50, q);
+                    const int e = std::min(N, s0 + lens[L]);
+                    if (pre[e] - pre[s0] >= 3) h.frag.push_back(SeedDesc{s0, lens[L]});
+                }
+            }
+        }
+        info.push_back(h.info);
+        hp_.push_back(std::move(h));
+        off += N;
+    }
+    A_.px = upload(b_px, px); A_.py = upload(b_py, py); A_.pz = upload(b_pz, pz);
+    A_.qx = upload(b_qx, qx); A_.qy = upload(b_qy, qy); A_.qz = upload(b_qz, qz);
+    A_.v = upload(b_v, v); A_.pv = upload(b_pv, pv); A_.nv = upload(b_nv, nv);
+    A_.pcx = upload(b_pcx, pcx); A_.pcy = upload(b_pcy, pcy); A_.pcz = upload(b_pcz, pcz);
+    A_.qcx = upload(b_qcx, qcx); A_.qcy = upload(b_qcy, qcy); A_.qcz = upload(b_qcz, qcz);
+    A_.pmap = upload(b_pmap, pmap); A_.nmap = upload(b_nmap, nmap);
+    A_.ssp = b_ssp.get<char>(std::max(off, 1));
+    A_.ssn = b_ssn.get<char>(std::max(off, 1));
+    A_.info = upload(b_info, info);
+    IA_CHECK(hipStreamSynchronize(s_));
+    best_.assign(hp_.size(), identity_hit(kNeg));
+}
+
+std::vector<int> Engine::max_seed_table(int nmax, int ms) {
+    std::vector<int> t(nmax + 1, 0);
+    int m = 0;
+    for (int n = 0; n <= nmax; n++) {
+        m = std::max(m, implicit_seed_count(n, ms));
+        t[n] = m;
+    }
+    return t;
+}
+
+Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc> &seeds,
+                       size_t n_out, Hit *out_dev) {
+    std::vector<int> blk_bank, blk_first;
+    long long cand = 0;
+    for (size_t b = 0; b < banks.size(); b++) {
+        banks[b].cand_off = cand;
+        cand += banks[b].cand_cap;
+        for (int j = 0; j < banks[b].cand_cap; j += 256) {
+            blk_bank.push_back((int)b);
+            blk_first.push_back(j);
+        }
+        stats[1] += banks[b].cand_cap;
+    }
+    if (!out_dev) out_dev = b_bank_out.get<Hit>(n_out);
+    if (banks.empty()) return out_dev;
+    const BankDesc *db = upload(b_banks, banks);
+    const int *dbb = upload(b_blk_bank, blk_bank);
+    const int *dbf = upload(b_blk_first, blk_first);
+    const SeedDesc *ds = upload(b_seeds, seeds);
+    Hit *dc = b_cands.get<Hit>(cand);
+    IA_CHECK(hipEventRecord(e0_, s_));
+    launch_bank(s_, A_, db, dbb, dbf, (int)blk_bank.size(), ds, b_masks.get<uint8_t>(1),
+                b_lists_p.get<int>(1), b_lists_n.get<int>(1), b_list_na.get<int>(1), list_cap_,
+                dc);
+    launched();
+    IA_CHECK(hipGetLastError());
+    IA_CHECK(hipEventRecord(e1_, s_));
+    launch_bank_reduce(s_, db, (int)banks.size(), b_list_na.get<int>(1), dc, out_dev);
+    launched();
+    IA_CHECK(hipGetLastError());
+    IA_CHECK(hipEventSynchronize(e1_));
+    float ms = 0;
+    IA_CHECK(hipEventElapsedTime(&ms, e0_, e1_));
+    stats[6] += ms;
+    return out_dev;
+}
+
+long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev, Hit *out_dev) {
+    long long total = 0;
+    int mp = 1, mn = 1, lc = 1;
+    for (ChainGroup &g : groups) {
+        g.chain_off = total;
+        total += g.n_chains;
+        const HostPair &h = hp_[g.pair];
+        mp = std::max(mp, h.plen);
+        mn = std::max(mn, h.nlen);
+        lc = std::max(lc, std::min(h.plen, h.nlen));
+        stats[3] += g.n_chains;
+        stats[2] += (double)h.plen * h.nlen * g.n_chains * g.iters;
+    }
+    if (total == 0) return 0;
+    const ChainGroup *dg = upload(b_groups, groups);
+    int *q = b_queue.get<int>(1);
+    IA_CHECK(hipMemsetAsync(q, 0, sizeof(int), s_));
+    const int wpr = (mn + 15) / 16;
+    const size_t trace_bytes = (size_t)mp * wpr * 4;
+    const bool in_lds = trace_bytes <= 96 * 1024;
+    const size_t lds = chain_lds_bytes(mp, mn, in_lds);
+    int per_cu = std::max(1, chain_blocks_per_cu(lds));
+    per_cu = std::min(per_cu, 16);
+    long long slots = (long long)n_cu_ * per_cu;
+    unsigned *trace = nullptr;
+    long long slot_words = 0;
+    if (!in_lds) {
+        slot_words = (long long)mp * wpr;
+        const long long budget = 8LL << 30;  // bytes of global trace slots
+        slots = std::max(1LL, std::min(slots, budget / (slot_words * 4)));
+        trace = b_trace.get<unsigned>((size_t)(slots * slot_words));
+    }
+    slots = std::min<long long>(slots, total);
+    IA_CHECK(hipEventRecord(e0_, s_));
+    launch_chains(s_, A_, dg, (int)groups.size(), total, xf_dev, out_dev,
+                  b_lout.get<ListOut>(1), b_lists_p.get<int>(1), b_lists_n.get<int>(1),
+                  b_list_na.get<int>(1), list_cap_, mp, mn, q, trace, slot_words, (int)slots);
+    launched();
+    IA_CHECK(hipGetLastError());
+    IA_CHECK(hipEventRecord(e1_, s_));
+    IA_CHECK(hipEventSynchronize(e1_));
+    float ms = 0;
+    IA_CHECK(hipEventElapsedTime(&ms, e0_, e1_));
+    stats[5] += ms;
+    return total;
+}
+
+std::vector<Hit> Engine::rescore(const std::vector<int> &pairs, const std::vector<Hit> &in) {
+    if (pairs.empty()) return {};
+    const int *dp = upload(b_rs_pair, pairs);
+    const Hit *di = upload(b_rs_in, in);
+    Hit *dout = b_rs_out.get<Hit>(in.size());
+    launch_rescore(s_, A_, dp, di, (int)in.size(), dout);
+    launched();
+    IA_CHECK(hipGetLastError());
+    return download(dout, in.size());
+}
+
+std::vector<Hit> Engine::group_reduce(const std::vector<long long> &offs,
+                                      const std::vector<int> &cnts, const Hit *in_dev) {
+    if (offs.empty()) return {};
+    const long long *d_off = upload(b_red_off, offs);
+    const int *d_cnt = upload(b_red_cnt, cnts);
+    Hit *dout = b_red_out.get<Hit>(offs.size());
+    launch_group_reduce(s_, d_off, d_cnt, (int)offs.size(), in_dev, dout);
+    launched();
+    IA_CHECK(hipGetLastError());
+    return download(dout, offs.size());
+}
+
+std::vector<Hit> Engine::dp_refine(const std::vector<int> &pairs, const std::vector<Hit> &start,
+                                   int iters, bool lite) {
+    const size_t n = pairs.size();
+    std::vector<Hit> res(n, identity_hit(0.0));  // align.h:61-64
+    std::vector<int> run;                        // entries with plen, nlen >= 3
+    for (size_t e = 0; e < n; e++)
+        if (hp_[pairs[e]].plen >= 3 && hp_[pairs[e]].nlen >= 3 && iters > 0) run.push_back((int)e);
+    if (run.empty()) return res;
+    if (lite) {
+        std::vector<ChainGroup> groups;
+        std::vector<Hit> xf;
+        for (int e : run) {
+            ChainGroup g{};
+            g.pair = pairs[e];
+            g.init = CI_XFORM;
+            g.mode = CM_LITE;
+            g.smat = SM_TM;
+            g.n_chains = 2;
+            g.iters = iters;
+            g.xf_off = (int)xf.size();
+            g.xf_stride = 1;
+            g.out_off = 2LL * (long long)xf.size();
+            g.out_stride = 2;
+            g.sd8 = hp_[pairs[e]].info.sd8;
+            groups.push_back(g);
+            xf.push_back(start[e]);
+        }
+        const Hit *dxf = upload(b_xf, xf);
+        Hit *dout = b_chain_out.get<Hit>(2 * xf.size());
+        run_chains(groups, dxf, dout);
+        std::vector<long long> offs;
+        std::vector<int> cnts;
+        for (size_t k = 0; k < xf.size(); k++) {
+            offs.push_back(2LL * k);
+            cnts.push_back(2);
+        }
+        const std::vector<Hit> m = group_reduce(offs, cnts, dout);
+        for (size_t k = 0; k < run.size(); k++) res[run[k]] = m[k];
+        return res;
+    }
+    // non-lite: per iteration, NW lists for both chains, full detailed search
+    // banks over the lists, then the merge of align.h:114-141 on the host.
+    const size_t nc = 2 * run.size();
+    std::vector<Hit> cur(nc), cb(nc, identity_hit(0.0));
+    for (size_t k = 0; k < run.size(); k++) cur[2 * k] = cur[2 * k + 1] = start[run[k]];
+    int lmax = 1, nmax = 1;
+    for (int e : run) {
+        lmax = std::max(lmax, std::min(hp_[pairs[e]].plen, hp_[pairs[e]].nlen));
+        nmax = std::max(nmax, hp_[pairs[e]].N);
+    }
+    const std::vector<int> kmax = max_seed_table(lmax, 50);
+    for (int it = 0; it < iters; it++) {
+        ensure_lists((int)nc, lmax);
+        std::vector<ChainGroup> groups;
+        for (size_t k = 0; k < run.size(); k++) {
+            ChainGroup g{};
+            g.pair = pairs[run[k]];
+            g.init = CI_XFORM;
+            g.mode = CM_LIST;
+            g.smat = SM_TM;
+            g.n_chains = 2;
+            g.iters = 1;
+            g.xf_off = (int)(2 * k);
+            g.xf_stride = 2;
+            g.xf_per_chain = 1;
+            g.out_off = 2LL * (long long)k;
+            g.out_stride = 2;
+            g.list_slot0 = (int)(2 * k);
+            g.sd8 = hp_[g.pair].info.sd8;
+            groups.push_back(g);
+        }
+        const Hit *dxf = upload(b_xf, cur);
+        Hit *dout = b_chain_out.get<Hit>(nc);
+        run_chains(groups, dxf, dout);
+        std::vector<BankDesc> banks;
+        for (size_t ch = 0; ch < nc; ch++) {
+            const HostPair &h = hp_[pairs[run[ch / 2]]];
+            banks.push_back(list_bank(pairs[run[ch / 2]], (int)ch,
+                                      kmax[std::min(h.plen, h.nlen)] * 6, h.info.sd8, (long long)ch));
+        }
+        Hit *ddet = run_banks(banks, {}, nc);
+        const std::vector<Hit> det = download(ddet, nc);
+        const std::vector<ListOut> lo = download(b_lout.get<ListOut>(nc), nc);
+        for (size_t ch = 0; ch < nc; ch++) {
+            if (lo[ch].na < 3) continue;  // align.h:106
+            Hit step;
+            std::memcpy(step.R, lo[ch].R, 72);
+            std::memcpy(step.t, lo[ch].t, 24);
+            step.score = lo[ch].it_score;
+            if (det[ch].score > step.score) step = det[ch];
+            cur[ch] = step;
+            if (step.score > cb[ch].score) cb[ch] = step;
+        }
+    }
+    for (size_t k = 0; k < run.size(); k++)
+        res[run[k]] = cb[2 * k + 1].score > cb[2 * k].score ? cb[2 * k + 1] : cb[2 * k];
+    return res;
+}
+
+
+void Engine::profile(const char *stage, std::chrono::steady_clock::time_point t0) {
+    static const bool on = std::getenv("INDIALIGN_PROFILE") != nullptr;
+    if (!on) return;
+    const char *tag = std::getenv("INDIALIGN_PROFILE_TAG");
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "[indialign]%s pairs=%zu stage=%s ms=%.3f\n", tag ? tag : "", hp_.size(),
+                 stage, ms);
+}
+
+// 1. seeds (seeds.h:41-82 on the host, anchors seeds.h:164-283 on the device)
+//    and the main seed bank (indialign.cpp:69-104).
+void Engine::stage_seed_bank() {
+    auto t0 = std::chrono::steady_clock::now();
+    const int n = (int)hp_.size();
+    std::vector<int> pairs(n);
+    std::vector<SeedDesc> frag;
+    std::vector<int> foff(n), fcnt(n);
+    std::vector<long long> aoff(n), soff(n);
+    long long abytes = 0, sints = 0;
+    for (int a = 0; a < n; a++) {
+        pairs[a] = a;
+        foff[a] = (int)frag.size();
+        fcnt[a] = (int)hp_[a].frag.size();
+        if (fcnt[a] > 990) throw DevError("too many fragment seeds for one pair");
+        frag.insert(frag.end(), hp_[a].frag.begin(), hp_[a].frag.end());
+        aoff[a] = abytes;
+        abytes += 30LL * hp_[a].N;
+        soff[a] = sints;
+        sints += 4LL * hp_[a].N;
+    }
+    const int *dpairs = upload(b_pairs, pairs);
+    const SeedDesc *dfrag = upload(b_seeds, frag);
+    const int *dfo = upload(b_fofs, foff);
+    const int *dfc = upload(b_fcnt, fcnt);
+    const long long *dao = upload(b_aoff, aoff);
+    const long long *dso = upload(b_soff, soff);
+    uint8_t *masks = b_masks.get<uint8_t>(abytes);
+    int *dac = b_acnt.get<int>(n);
+    launch_anchors(s_, A_, dpairs, n, dfrag, dfo, dfc, masks, dao, dac, b_ascr.get<int>(sints), dso);
+    launched();
+    IA_CHECK(hipGetLastError());
+    const std::vector<int> acnt = download(dac, n);
+    std::vector<SeedDesc> seeds;
+    std::vector<BankDesc> banks;
+    for (int a = 0; a < n; a++) {
+        const HostPair &h = hp_[a];
+        const int K = fcnt[a] + acnt[a];
+        if (K == 0) continue;  // indialign.cpp:77-81: score 0, identity
+        BankDesc b{};
+        b.pair = a;
+        b.pts = PTS_FULL;
+        b.n = h.N;
+        b.seeds = SEED_EXPLICIT;
+        b.seed_off = (int)seeds.size();
+        b.K = K;
+        b.C = h.info.C;
+        b.iters = cfg_->max_iter;
+        b.use_d8 = 0;
+        b.init = INIT_SEED;
+        b.cand_cap = K * h.info.C;
+        b.out_idx = a;
+        b.et = serial_ ? cfg_->early_term_score : kNever;
+        b.sd8 = h.info.sd8;
+        std::memcpy(b.cut, h.info.cut, sizeof(b.cut));
+        banks.push_back(b);
+        seeds.insert(seeds.end(), h.frag.begin(), h.frag.end());
+        for (int r = 0; r < acnt[a]; r++) seeds.push_back(SeedDesc{(int)(aoff[a] + (long long)r * h.N), -1});
+        live_.push_back(a);
+    }
+    Hit *d = run_banks(banks, seeds, n);
+    const std::vector<Hit> res = download(d, n);
+    for (int a : live_) best_[a] = res[a];
+    for (int a = 0; a < n; a++)
+        if (fcnt[a] + acnt[a] == 0) best_[a] = identity_hit(0.0);
+    profile("seed_bank", t0);
+}
+
+// 2b. secondary-structure seeds IA2 + IA4 (ss.h:87-164), rescored
+//     (indialign.cpp:106-116).
+void Engine::stage_ss() {
+    auto t0 = std::chrono::steady_clock::now();
+    const int dp_iter = cfg_->dp_iter;
+    std::vector<Hit> ssb(hp_.size(), identity_hit(kNeg));
+    std::vector<int> E;
+    int lmax = 1;
+    for (int a : live_)
+        if (hp_[a].plen >= 3 && hp_[a].nlen >= 3) {
+            E.push_back(a);
+            lmax = std::max(lmax, std::min(hp_[a].plen, hp_[a].nlen));
+        }
+    if (!E.empty()) {
+        launch_ss(s_, A_, (int)hp_.size(), (char *)A_.ssp, (char *)A_.ssn);
+        launched();
+        IA_CHECK(hipGetLastError());
+        const std::vector<int> kmax = max_seed_table(lmax, 50);
+        for (int pass = 0; pass < 2; pass++) {  // 0 = IA2, 1 = IA4
+            ensure_lists((int)E.size(), lmax);
+            std::vector<ChainGroup> groups;
+            std::vector<Hit> xf;
+            for (size_t e = 0; e < E.size(); e++) {
+                const int a = E[e];
+                ChainGroup g{};
+                g.pair = a;
+                g.init = CI_XFORM;
+                g.mode = CM_LIST;
+                g.smat = pass == 0 ? SM_SS : SM_IA4;
+                g.n_chains = 1;
+                g.iters = 1;
+                g.xf_off = (int)e;
+                g.xf_stride = 1;
+                g.out_off = (long long)e;
+                g.out_stride = 1;
+                g.list_slot0 = (int)e;
+                g.sd8 = hp_[a].info.sd8;
+                g.gap_ss = -1.0;  // ss.h:75,144
+                groups.push_back(g);
+                // IA4 starts from the IA2 result when there is one (ss.h:114-115)
+                xf.push_back(pass == 0 ? identity_hit(0.0) : (ssb[a].score > -1e20 ? ssb[a] : best_[a]));
+            }
+            const Hit *dxf = upload(b_xf, xf);
+            run_chains(groups, dxf, b_chain_out.get<Hit>(E.size()));
+            std::vector<BankDesc> banks;
+            for (size_t e = 0; e < E.size(); e++) {
+                const HostPair &h = hp_[E[e]];
+                banks.push_back(list_bank(E[e], (int)e, kmax[std::min(h.plen, h.nlen)] * 6, h.info.sd8, (long long)e));
+            }
+            Hit *dd = run_banks(banks, {}, E.size());
+            const std::vector<Hit> det = download(dd, E.size());
+            const std::vector<ListOut> lo = download(b_lout.get<ListOut>(E.size()), E.size());
+            std::vector<int> dpp;
+            std::vector<Hit> dps;
+            for (size_t e = 0; e < E.size(); e++) {
+                const int a = E[e];
+                if (lo[e].na < 3) continue;
+                if (det[e].score > ssb[a].score) ssb[a] = det[e];
+                if (dp_iter > 0) {
+                    dpp.push_back(a);
+                    dps.push_back(ssb[a]);
+                }
+            }
+            if (!dpp.empty()) {
+                const std::vector<Hit> dp = dp_refine(dpp, dps, dp_iter, false);
+                for (size_t k = 0; k < dpp.size(); k++)
+                    if (dp[k].score > ssb[dpp[k]].score) ssb[dpp[k]] = dp[k];
+            }
+        }
+    }
+    std::vector<Hit> in;
+    for (int a : live_) in.push_back(ssb[a]);
+    const std::vector<Hit> rs = rescore(live_, in);
+    for (size_t k = 0; k < live_.size(); k++) offer(live_[k], rs[k]);
+    profile("ss", t0);
+}
+
+// 2c. gapless threading (threading.h:41-80), same-index scores.
+void Engine::stage_gapless() {
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<BankDesc> banks;
+    std::vector<SeedDesc> seeds;
+    std::vector<int> who;
+    for (int a : live_) {
+        const HostPair &h = hp_[a];
+        const int min_ali = std::max(3, std::min(h.plen, h.nlen) / 2);
+        const int kr = h.N - min_ali;
+        if (kr <= 0) continue;
+        const int stride = std::max(1, (2 * kr + 99) / 100);
+        BankDesc b{};
+        b.pair = a;
+        b.pts = PTS_FULL;
+        b.n = h.N;
+        b.seeds = SEED_EXPLICIT;
+        b.seed_off = (int)seeds.size();
+        for (int k = -kr; k <= kr; k += stride)
+            if (k != 0) seeds.push_back(SeedDesc{k, -2});
+        b.K = (int)seeds.size() - b.seed_off;
+        b.C = 1;
+        b.cut[0] = h.info.d0s;
+        b.iters = 5;
+        b.use_d8 = 0;
+        b.init = INIT_OFFSET;
+        b.min_ali = min_ali;
+        b.cand_cap = b.K;
+        b.out_idx = (long long)who.size();
+        b.et = kNever;
+        b.sd8 = h.info.sd8;
+        if (b.K == 0) continue;
+        banks.push_back(b);
+        who.push_back(a);
+    }
+    if (!banks.empty()) {
+        Hit *d = run_banks(banks, seeds, who.size());
+        const std::vector<Hit> res = download(d, who.size());
+        for (size_t k = 0; k < who.size(); k++) offer(who[k], res[k]);
+    }
+    profile("gapless_threading", t0);
+}
+
+// 3. rescue passes (indialign.cpp:125-152; rescue.h).
+void Engine::stage_rescue() {
+    const double rs1 = cfg_->rescue_score_1, rs2 = cfg_->rescue_score_2;
+    auto local_frag = [&](bool dense, const char *name) {
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<int> P;
+        for (int a : live_)
+            if (best_[a].score < (dense ? rs2 : rs1)) P.push_back(a);
+        if (P.empty()) return;
+        std::vector<ChainGroup> groups;
+        std::vector<long long> offs;
+        std::vector<int> cnts;
+        long long base = 0;
+        for (int a : P) {
+            const HostPair &h = hp_[a];
+            ChainGroup g{};
+            g.pair = a;
+            g.init = CI_FRAG;
+            g.mode = CM_LITE;
+            g.smat = SM_TM;
+            g.iters = 2;        // rescue.h:92-94
+            g.sd8 = 100.0;
+            g.jp = rescue_stride(h.plen, dense);
+            g.jn = rescue_stride(h.nlen, dense);
+            int items = 0;
+            for (int len : rescue_lengths(h.plen, h.nlen, dense)) {
+                const int pmax = h.plen - len + 1, nmax = h.nlen - len + 1;
+                if (pmax <= 0 || nmax <= 0) continue;
+                const int nps = (pmax + g.jp - 1) / g.jp, nns = (nmax + g.jn - 1) / g.jn;
+                g.lens[g.n_lens] = len;
+                g.items_prefix[g.n_lens] = items;
+                g.n_ns[g.n_lens] = nns;
+                g.n_lens++;
+                items += nps * nns;
+            }
+            g.items_prefix[g.n_lens] = items;
+            g.n_chains = 2 * items;
+            g.out_off = base;
+            g.out_stride = 2;
+            offs.push_back(base);
+            cnts.push_back(2 * items);
+            if (items > 0) groups.push_back(g);
+            base += 2LL * items;
+        }
+        Hit *dout = b_chain_out.get<Hit>(base);
+        run_chains(groups, nullptr, dout);
+        const std::vector<Hit> red = group_reduce(offs, cnts, dout);
+        const std::vector<Hit> rs = rescore(P, red);
+        for (size_t k = 0; k < P.size(); k++) offer(P[k], rs[k]);
+        profile(name, t0);
+    };
+    local_frag(false, "local_fragment");
+
+    {   // threading DP (rescue.h:121-190)
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<int> P;
+        for (int a : live_)
+            if (best_[a].score < rs1) P.push_back(a);
+        if (!P.empty()) {
+            std::vector<BankDesc> banks;
+            std::vector<ChainGroup> groups;
+            std::vector<long long> offs;
+            std::vector<int> cnts;
+            long long base = 0;
+            static const double mult[6] = {2.0, 1.5, 1.0, 0.75, 0.5, 0.25};
+            for (int a : P) {
+                const HostPair &h = hp_[a];
+                const int pl = h.plen, nl = h.nlen;
+                int no = 0;
+                if (std::min(pl, nl) >= 3) {
+                    const int step = std::max(1, std::min(pl, nl) / 50);
+                    for (int off = -(nl - 3); off < pl - 2; off += step) {
+                        const int ps = std::max(off, 0), ns = std::max(-off, 0);
+                        const int ov = std::min(pl - ps, nl - ns);
+                        if (ov < 3) continue;
+                        BankDesc b{};
+                        b.pair = a;
+                        b.pts = PTS_CWIN;
+                        b.n = ov;
+                        b.a0 = ps;
+                        b.a1 = ns;
+                        b.seeds = SEED_IMPLICIT;
+                        b.max_starts = 5;  // rescue.h:159-161
+                        b.C = 6;
+                        for (int c = 0; c < 6; c++) b.cut[c] = h.info.d0s * mult[c];
+                        b.iters = 8;
+                        b.use_d8 = 1;
+                        b.sd8 = 100.0;
+                        b.init = INIT_SEED;
+                        b.et = 1.0;
+                        b.cand_cap = implicit_seed_count(ov, 5) * 6;
+                        b.out_idx = base + 3LL * no;
+                        banks.push_back(b);
+                        no++;
+                    }
+                }
+                if (no > 0) {
+                    ChainGroup g{};
+                    g.pair = a;
+                    g.init = CI_XFORM;
+                    g.mode = CM_LITE;
+                    g.smat = SM_TM;
+                    g.iters = 1;
+                    g.sd8 = 100.0;
+                    g.n_chains = 2 * no;
+                    g.xf_off = (int)base;
+                    g.xf_stride = 3;
+                    g.out_off = base + 1;
+                    g.out_stride = 3;
+                    groups.push_back(g);
+                }
+                offs.push_back(base);
+                cnts.push_back(3 * no);
+                base += 3LL * no;
+            }
+            Hit *dthr = b_chain_out.get<Hit>(base);
+            run_banks(banks, {}, base, dthr);
+            run_chains(groups, dthr, dthr);
+            const std::vector<Hit> red = group_reduce(offs, cnts, dthr);
+            const std::vector<Hit> rs = rescore(P, red);
+            for (size_t k = 0; k < P.size(); k++) offer(P[k], rs[k]);
+            profile("threading_dp", t0);
+        }
+    }
+    local_frag(true, "dense_local_fragment");
+}
+
+// 4. top-k distance seed (indialign.cpp:154-177).
+void Engine::stage_topk() {
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<int> P;
+    std::vector<Hit> bst;
+    std::vector<long long> moff;
+    long long m = 0;
+    for (int a : live_)
+        if (hp_[a].V >= 3) {
+            P.push_back(a);
+            bst.push_back(best_[a]);
+            moff.push_back(m);
+            m += hp_[a].N;
+        }
+    if (!P.empty()) {
+        const int *dp = upload(b_pairs, P);
+        const Hit *db = upload(b_xf, bst);
+        const long long *dm = upload(b_moff, moff);
+        uint8_t *masks = b_masks.get<uint8_t>(m);
+        launch_topk(s_, A_, dp, db, (int)P.size(), dm, masks);
+        launched();
+        IA_CHECK(hipGetLastError());
+        std::vector<BankDesc> banks;
+        std::vector<SeedDesc> seeds;
+        for (size_t k = 0; k < P.size(); k++) {
+            const HostPair &h = hp_[P[k]];
+            BankDesc b{};
+            b.pair = P[k];
+            b.pts = PTS_FULL;
+            b.n = h.N;
+            b.seeds = SEED_EXPLICIT;
+            b.seed_off = (int)seeds.size();
+            seeds.push_back(SeedDesc{(int)moff[k], -1});
+            b.K = 1;
+            b.C = h.info.C;
+            std::memcpy(b.cut, h.info.cut, sizeof(b.cut));
+            b.iters = cfg_->max_iter;
+            b.use_d8 = 0;
+            b.init = INIT_SEED;
+            b.cand_cap = h.info.C;
+            b.out_idx = (long long)k;
+            b.et = cfg_->early_term_score;
+            b.sd8 = h.info.sd8;
+            banks.push_back(b);
+        }
+        Hit *d = run_banks(banks, seeds, P.size());
+        const std::vector<Hit> res = download(d, P.size());
+        for (size_t k = 0; k < P.size(); k++) offer(P[k], res[k]);
+    }
+    profile("topk_seed", t0);
+}
+
+// 5. DP refinement and 5b. raw offset starts (indialign.cpp:179-256).
+void Engine::stage_dp() {
+    auto t0 = std::chrono::steady_clock::now();
+    const int dp_iter = cfg_->dp_iter;
+    if (dp_iter <= 0) {
+        std::vector<Hit> in;
+        for (int a : live_) in.push_back(best_[a]);
+        const std::vector<Hit> rs = rescore(live_, in);
+        for (size_t k = 0; k < live_.size(); k++) offer(live_[k], rs[k]);
+        profile("dp_refine", t0);
+        return;
+    }
+    std::vector<double> seed_score(hp_.size());
+    std::vector<Hit> start;
+    for (int a : live_) {
+        seed_score[a] = best_[a].score;
+        start.push_back(best_[a]);
+    }
+    const std::vector<Hit> dp = dp_refine(live_, start, dp_iter, false);
+    const std::vector<Hit> dp_id = rescore(live_, dp);
+    for (size_t k = 0; k < live_.size(); k++) offer(live_[k], dp_id[k]);
+    {
+        std::vector<Hit> cur;
+        for (int a : live_) cur.push_back(best_[a]);
+        const std::vector<Hit> orig = rescore(live_, cur);
+        for (size_t k = 0; k < live_.size(); k++) offer(live_[k], orig[k]);
+    }
+    std::vector<int> X;
+    std::vector<size_t> xk;
+    for (size_t k = 0; k < live_.size(); k++)
+        if (hp_[live_[k]].cross) {
+            X.push_back(live_[k]);
+            xk.push_back(k);
+        }
+    if (!X.empty()) {
+        const std::vector<Hit> idp = dp_refine(X, std::vector<Hit>(X.size(), identity_hit(0.0)), dp_iter, true);
+        const std::vector<Hit> rs = rescore(X, idp);
+        for (size_t k = 0; k < X.size(); k++) offer(X[k], rs[k]);
+        const int extra = std::max(0, 5 - dp_iter);
+        if (extra > 0) {
+            std::vector<int> Y;
+            std::vector<Hit> from;
+            for (size_t k = 0; k < X.size(); k++) {
+                const int a = X[k];
+                if (!(best_[a].score < 0.36)) continue;
+                Y.push_back(a);
+                const size_t lk = xk[k];
+                from.push_back(seed_score[a] > dp_id[lk].score ? best_[a] : dp[lk]);
+            }
+            if (!Y.empty()) {
+                const std::vector<Hit> ex = dp_refine(Y, from, extra, true);
+                const std::vector<Hit> rs2 = rescore(Y, ex);
+                for (size_t k = 0; k < Y.size(); k++) offer(Y[k], rs2[k]);
+            }
+        }
+    }
+    profile("dp_refine", t0);
+    t0 = std::chrono::steady_clock::now();
+    // offsets k = -N/2 .. N/2 step max(1, N/10), k != 0, >= 3 offset pairs
+    std::vector<ChainGroup> groups;
+    std::vector<long long> offs;
+    std::vector<int> cnts, owner;
+    long long base = 0;
+    for (int a : live_) {
+        const HostPair &h = hp_[a];
+        if (h.plen < 3 || h.nlen < 3) continue;
+        const int kstep = std::max(1, h.N / 10), k0 = -h.N / 2;
+        int count = 0;
+        for (int k = k0; k <= h.N / 2; k += kstep) count++;
+        ChainGroup g{};
+        g.pair = a;
+        g.init = CI_OFFSET;
+        g.mode = CM_LITE;
+        g.smat = SM_TM;
+        g.iters = dp_iter;
+        g.sd8 = h.info.sd8;
+        g.k0 = k0;
+        g.kstep = kstep;
+        g.n_chains = 2 * count;
+        g.out_off = base;
+        g.out_stride = 2;
+        groups.push_back(g);
+        for (int o = 0; o < count; o++) {
+            offs.push_back(base + 2LL * o);
+            cnts.push_back(2);
+            owner.push_back(a);
+        }
+        base += 2LL * count;
+    }
+    if (!groups.empty()) {
+        Hit *dout = b_chain_out.get<Hit>(base);
+        run_chains(groups, nullptr, dout);
+        const std::vector<Hit> red = group_reduce(offs, cnts, dout);
+        const std::vector<Hit> rs = rescore(owner, red);
+        for (size_t k = 0; k < owner.size(); k++)
+            if (rs[k].score != -INFINITY) offer(owner[k], rs[k]);  // skipped offsets
+    }
+    profile("offset_dp", t0);
+}
+
+// 6. weighted TM refinement (indialign.cpp:258-264).
+void Engine::stage_weighted() {
+    auto t0 = std::chrono::steady_clock::now();
+    if (live_.empty()) return;
+    std::vector<Hit> in;
+    for (int a : live_) in.push_back(best_[a]);
+    const int *dp = upload(b_pairs, live_);
+    const Hit *di = upload(b_xf, in);
+    Hit *dout = b_rs_out.get<Hit>(in.size());
+    launch_weighted(s_, A_, dp, di, (int)in.size(), dout);
+    launched();
+    IA_CHECK(hipGetLastError());
+    const std::vector<Hit> w = download(dout, in.size());
+    for (size_t k = 0; k < live_.size(); k++) offer(live_[k], w[k]);
+    profile("weighted_refine", t0);
+}
+
+int Engine::run(const NativePairInput *inp, int n, const NativeConfig *cfg, bool serial_bank,
+                NativePairResult *out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (double &x : stats) x = 0;
+    cfg_ = cfg;
+    serial_ = serial_bank;
+    live_.clear();
+    // Lnorm <= 2 -> score 0, identity (indialign.cpp:47-51)
+    for (int k = 0; k < n; k++) {
+        if (inp[k].length <= 0 || inp[k].lnorm > 2.0) continue;
+        out[k].score = 0.0;
+        set_identity(out[k].rotation, out[k].translation);
+    }
+    prepare(inp, n);
+    if (!hp_.empty()) {
+        stage_seed_bank();
+        stage_ss();
+        stage_gapless();
+        stage_rescue();
+        stage_topk();
+        stage_dp();
+        stage_weighted();
+        for (size_t a = 0; a < hp_.size(); a++) {
+            NativePairResult &o = out[hp_[a].src];
+            o.score = best_[a].score;
+            std::memcpy(o.rotation, best_[a].R, 72);
+            std::memcpy(o.translation, best_[a].t, 24);
+        }
+    }
+    stats[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+}  // namespace ia
+
+// ---- C ABI ----------------------------------------------------------------
+namespace {
+std::mutex g_mu;
+std::unique_ptr<ia::Engine> g_engine;
+double g_stats[8];
+
+int run_search(int n, const NativePairInput *inp, const NativeConfig *cfg, NativePairResult *out,
+               int serial) {
+    const char *env = std::getenv("INDIALIGN_SERIAL_BANK");
+    if (env && *env) serial = std::atoi(env) != 0;
+    std::lock_guard<std::mutex> lk(g_mu);
+    try {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+            throw ia::DevError("no HIP device available");
+        if (!g_engine) g_engine.reset(new ia::Engine());
+        const int rc = g_engine->run(inp, n, cfg, serial != 0, out);
+        std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
+        return rc;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "indialign_amd: %s\n", e.what());
+        g_engine.reset();
+        return -2;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+void indialign_default_config(NativeConfig *cfg) {
+    cfg->backend_mode = 0;
+    cfg->use_fragment_search = 1;
+    cfg->max_iter = 20;
+    cfg->dp_iter = 1;
+    cfg->max_mem_gb = 20.0;
+    cfg->request_cuda = 0;
+    cfg->rescue_score_1 = 0.5;
+    cfg->rescue_score_2 = 0.3;
+    cfg->early_term_score = 0.99;
+}
+
+int indialign_tmscore_search(const NativePairInput *inp, const NativeConfig *cfg,
+                             NativePairResult *out) {
+    if (inp->length <= 0) return -1;
+    return run_search(1, inp, cfg, out, 0);
+}
+
+int indialign_tmscore_search_batch(int n_pairs, const NativePairInput *inputs,
+                                   const NativeConfig *cfg, NativePairResult *outputs) {
+    if (n_pairs <= 0) return 0;
+    return run_search(n_pairs, inputs, cfg, outputs, 1);
+}
+
+int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
+                        NativePairResult *outputs, int serial_bank) {
+    if (n_pairs <= 0) return 0;
+    return run_search(n_pairs, inputs, cfg, outputs, serial_bank);
+}
+
+int indialign_cuda_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n > 0 ? 1 : 0;
+}
+
+int indialign_last_stats(double *out, int n) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    const int m = n < 8 ? n : 8;
+    for (int k = 0; k < m; k++) out[k] = g_stats[k];
+    return m;
+}
+
+}  // extern "C"

@@ -1,0 +1,273 @@
+// Shared host/device definitions for the gfx950 TM-score superposition search.
+//
+// All FP64 expressions below are written in the evaluation order of the
+// reference (indialign_c/core.h) and the library is compiled with
+// -ffp-contract=off, so a lane that sums in index order reproduces the
+// reference's sequential FP64 result bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define IA_HD __host__ __device__ __forceinline__
+#define IA_D __device__ __forceinline__
+
+namespace ia {
+
+constexpr double kNeg = -1e30;          // "no result yet" (search.h:67, rescue.h:56)
+constexpr int kMaxCut = 11;             // 6 multipliers + 5 offsets (indialign.cpp:54-67)
+constexpr int kWave = 64;
+
+// One candidate transform + score: R[9] (out_d = sum_k P_k R[3k+d]), t[3], score.
+struct Hit {
+    double R[9];
+    double t[3];
+    double score;
+};
+
+// Per-pair constants, computed on the host once per call.
+struct PairInfo {
+    int N;          // residues
+    int off;        // element offset of this pair in the coordinate / mask arenas
+    int plen, nlen; // pred-valid / native-valid counts (compact DP sizes)
+    int V;          // valid count
+    int C;          // d0 search candidates
+    double Lnorm, d0, d0s, sd8;
+    double cut[kMaxCut];
+};
+
+// Coordinate arena: SoA FP64 for pred (p*) and native (q*), masks, compact
+// pred/native coordinates (pc*, qc*) and compact->full index maps.
+struct Arena {
+    const double *px, *py, *pz, *qx, *qy, *qz;
+    const uint8_t *v, *pv, *nv;
+    const double *pcx, *pcy, *pcz, *qcx, *qcy, *qcz;  // compact by pv / nv
+    const int *pmap, *nmap;                            // compact -> full index
+    const char *ssp, *ssn;                             // secondary structure (full index)
+    const PairInfo *info;
+};
+
+IA_HD void set_identity(double *R, double *t) {
+    R[0] = 1; R[1] = 0; R[2] = 0;
+    R[3] = 0; R[4] = 1; R[5] = 0;
+    R[6] = 0; R[7] = 0; R[8] = 1;
+    t[0] = t[1] = t[2] = 0;
+}
+
+// Rotated + translated point, reference order (core.h:127-131).
+IA_HD void xform(const double *R, const double *t, double x, double y, double z, double &mx,
+                 double &my, double &mz) {
+    mx = x * R[0] + y * R[3] + z * R[6] + t[0];
+    my = x * R[1] + y * R[4] + z * R[7] + t[1];
+    mz = x * R[2] + y * R[5] + z * R[8] + t[2];
+}
+
+IA_HD double dist2_moved(const double *R, const double *t, double x, double y, double z,
+                         double qx, double qy, double qz) {
+    double mx, my, mz;
+    xform(R, t, x, y, z, mx, my, mz);
+    const double dx = mx - qx, dy = my - qy, dz = mz - qz;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Cyclic Jacobi, 8 sweeps, pair order (0,1)(0,2)(0,3)(1,2)(1,3)(2,3),
+// rotation skipped when |a_pq| < 1e-15 (core.h:29-60).  Fully unrolled so
+// every index is a compile-time constant (the matrices stay in registers).
+template <int P, int Q>
+IA_HD void jacobi_rot(double a[4][4], double v[4][4]) {
+    const double apq = a[P][Q];
+    if (fabs(apq) < 1e-15) return;
+    const double theta = (a[Q][Q] - a[P][P]) / (2.0 * apq);
+    const double tn = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+    const double c = 1.0 / sqrt(1.0 + tn * tn);
+    const double s = tn * c;
+    a[P][P] -= tn * apq;
+    a[Q][Q] += tn * apq;
+    a[P][Q] = 0.0;
+    a[Q][P] = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        if (r == P || r == Q) continue;
+        const double xp = a[r][P], xq = a[r][Q];
+        a[r][P] = c * xp - s * xq;
+        a[P][r] = a[r][P];
+        a[r][Q] = s * xp + c * xq;
+        a[Q][r] = a[r][Q];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const double xp = v[r][P], xq = v[r][Q];
+        v[r][P] = c * xp - s * xq;
+        v[r][Q] = s * xp + c * xq;
+    }
+}
+
+IA_HD void jacobi4(double a[4][4], double v[4][4]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[i][j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 8; sweep++) {
+        jacobi_rot<0, 1>(a, v);
+        jacobi_rot<0, 2>(a, v);
+        jacobi_rot<0, 3>(a, v);
+        jacobi_rot<1, 2>(a, v);
+        jacobi_rot<1, 3>(a, v);
+        jacobi_rot<2, 3>(a, v);
+    }
+}
+
+// Horn key matrix from the 3x3 correlation h (row r = P axis, col c = Q
+// axis), dominant eigenvector (first maximum wins), quaternion -> R
+// (core.h:64-90).
+IA_HD void horn_rotation(const double h[9], double *R) {
+    double f[4][4];
+    f[0][0] = h[0] + h[4] + h[8];
+    f[0][1] = f[1][0] = h[5] - h[7];
+    f[0][2] = f[2][0] = h[6] - h[2];
+    f[0][3] = f[3][0] = h[1] - h[3];
+    f[1][1] = h[0] - h[4] - h[8];
+    f[1][2] = f[2][1] = h[1] + h[3];
+    f[1][3] = f[3][1] = h[6] + h[2];
+    f[2][2] = -h[0] + h[4] - h[8];
+    f[2][3] = f[3][2] = h[5] + h[7];
+    f[3][3] = -h[0] - h[4] + h[8];
+    double v[4][4];
+    jacobi4(f, v);
+    int top = 0;
+    double best = f[0][0];
+    if (f[1][1] > best) { top = 1; best = f[1][1]; }
+    if (f[2][2] > best) { top = 2; best = f[2][2]; }
+    if (f[3][3] > best) { top = 3; best = f[3][3]; }
+    double q0, q1, q2, q3;
+    if (top == 0) { q0 = v[0][0]; q1 = v[1][0]; q2 = v[2][0]; q3 = v[3][0]; }
+    else if (top == 1) { q0 = v[0][1]; q1 = v[1][1]; q2 = v[2][1]; q3 = v[3][1]; }
+    else if (top == 2) { q0 = v[0][2]; q1 = v[1][2]; q2 = v[2][2]; q3 = v[3][2]; }
+    else { q0 = v[0][3]; q1 = v[1][3]; q2 = v[2][3]; q3 = v[3][3]; }
+    const double nrm = sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+    if (nrm < 1e-12) {
+        q0 = 1.0; q1 = q2 = q3 = 0.0;
+    } else {
+        q0 /= nrm; q1 /= nrm; q2 /= nrm; q3 /= nrm;
+    }
+    R[0] = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
+    R[1] = 2 * (q1 * q2 + q0 * q3);
+    R[2] = 2 * (q1 * q3 - q0 * q2);
+    R[3] = 2 * (q1 * q2 - q0 * q3);
+    R[4] = q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3;
+    R[5] = 2 * (q2 * q3 + q0 * q1);
+    R[6] = 2 * (q1 * q3 + q0 * q2);
+    R[7] = 2 * (q2 * q3 - q0 * q1);
+    R[8] = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+}
+
+// t = cQ - cP.R (core.h:121-122).
+IA_HD void finish_t(const double *cp, const double *cq, const double *R, double *t) {
+    t[0] = cq[0] - (cp[0] * R[0] + cp[1] * R[3] + cp[2] * R[6]);
+    t[1] = cq[1] - (cp[0] * R[1] + cp[1] * R[4] + cp[2] * R[7]);
+    t[2] = cq[2] - (cp[0] * R[2] + cp[1] * R[5] + cp[2] * R[8]);
+}
+
+// Sequential two-pass Kabsch accumulators (core.h:94-123): feed the selected
+// points in index order to add_centroid, call centre(), feed them again to
+// add_cov, then solve().
+struct KabschSeq {
+    double cp[3], cq[3], cnt, h[9];
+    IA_HD void reset() {
+        cp[0] = cp[1] = cp[2] = cq[0] = cq[1] = cq[2] = 0.0;
+        cnt = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) h[k] = 0.0;
+    }
+    IA_HD void add_centroid(double x, double y, double z, double a, double b, double c) {
+        cp[0] += x; cp[1] += y; cp[2] += z;
+        cq[0] += a; cq[1] += b; cq[2] += c;
+        cnt += 1.0;
+    }
+    IA_HD void centre() {
+        cp[0] /= cnt; cp[1] /= cnt; cp[2] /= cnt;
+        cq[0] /= cnt; cq[1] /= cnt; cq[2] /= cnt;
+    }
+    IA_HD void add_cov(double x, double y, double z, double a, double b, double c) {
+        const double d0 = x - cp[0], d1 = y - cp[1], d2 = z - cp[2];
+        const double e0 = a - cq[0], e1 = b - cq[1], e2 = c - cq[2];
+        h[0] += d0 * e0; h[1] += d0 * e1; h[2] += d0 * e2;
+        h[3] += d1 * e0; h[4] += d1 * e1; h[5] += d1 * e2;
+        h[6] += d2 * e0; h[7] += d2 * e1; h[8] += d2 * e2;
+    }
+    IA_HD void solve(double *R, double *t) const {
+        horn_rotation(h, R);
+        finish_t(cp, cq, R, t);
+    }
+};
+
+// TM term 1/(1 + d2/d0sq) (core.h:147,163).
+IA_HD double tm_term(double d2, double d0sq) { return 1.0 / (1.0 + d2 / d0sq); }
+
+// ---- window seeds over n points (seeds.h:15-37), computed on the fly ----
+// Returns the number of distinct window lengths (descending) in lens.
+IA_HD int window_lengths(int V, int *lens) {
+    if (V < 4) return 0;
+    int c[9] = {V, V / 2, V / 4, V / 8, 64, 32, 16, 8, 4};
+    int n = 0;
+    // insertion into a descending unique list
+    for (int k = 0; k < 9; k++) {
+        const int x = c[k];
+        if (x < 4 || x > V) continue;
+        bool dup = false;
+        for (int j = 0; j < n; j++) dup |= lens[j] == x;
+        if (dup) continue;
+        int pos = n;
+        while (pos > 0 && lens[pos - 1] < x) {
+            lens[pos] = lens[pos - 1];
+            pos--;
+        }
+        lens[pos] = x;
+        n++;
+    }
+    return n;
+}
+
+IA_HD int window_start_count(int n, int len, int ms) {
+    const int total = n - len + 1;
+    if (total <= 0) return 0;
+    if (total <= ms) return total;
+    int step = total / ms;
+    if (step < 1) step = 1;
+    const int cnt = (total - 1) / step + 1;           // 0, step, ... < total
+    return cnt + (((cnt - 1) * step != total - 1) ? 1 : 0);
+}
+
+IA_HD int window_start(int n, int len, int ms, int idx) {
+    const int total = n - len + 1;
+    if (total <= ms) return idx;
+    int step = total / ms;
+    if (step < 1) step = 1;
+    const int cnt = (total - 1) / step + 1;
+    return idx < cnt ? idx * step : total - 1;
+}
+
+// Seeds over n all-valid points: k = 0 is the whole set, then the windows.
+// (All windows hold >= 4 points, so none is dropped by the >= 3 filter.)
+IA_HD int implicit_seed_count(int n, int ms) {
+    if (n < 3) return 0;
+    int lens[9];
+    const int nl = window_lengths(n, lens);
+    int K = 1;
+    for (int j = 0; j < nl; j++) K += window_start_count(n, lens[j], ms);
+    return K;
+}
+
+IA_HD void implicit_seed(int n, int ms, int k, int &start, int &len) {
+    if (k == 0) { start = 0; len = n; return; }
+    int lens[9];
+    const int nl = window_lengths(n, lens);
+    int r = k - 1;
+    for (int j = 0; j < nl; j++) {
+        const int c = window_start_count(n, lens[j], ms);
+        if (r < c) { start = window_start(n, lens[j], ms, r); len = lens[j]; return; }
+        r -= c;
+    }
+    start = 0; len = 0;
+}
+
+}  // namespace ia

@@ -1,0 +1,421 @@
+// Seed-bank refinement (K1) and the small per-pair kernels around it.
+//
+// K1 maps one candidate (seed k, d0 cutoff c) to one lane.  The 64 lanes of
+// a wavefront share the point set, so the per-point loop is uniform (its
+// coordinate loads are broadcast) while each lane keeps its own transform,
+// inlier count and convergence state.  Every sum runs in index order inside
+// the lane, which is the order of the reference loops, so a candidate's
+// (R, t, score) equals the FP64 CPU result bit for bit.
+#include "ia_kernels.h"
+
+#include <math.h>
+
+namespace ia {
+
+namespace {
+
+template <int PTS>
+IA_D void load_pt(const Arena &A, int off, const int *lp, const int *ln, int a0, int a1, int i,
+                  double &x, double &y, double &z, double &qa, double &qb, double &qc) {
+    if (PTS == PTS_FULL) {
+        x = A.px[off + i]; y = A.py[off + i]; z = A.pz[off + i];
+        qa = A.qx[off + i]; qb = A.qy[off + i]; qc = A.qz[off + i];
+    } else if (PTS == PTS_LIST) {
+        const int ip = off + lp[i], in = off + ln[i];
+        x = A.px[ip]; y = A.py[ip]; z = A.pz[ip];
+        qa = A.qx[in]; qb = A.qy[in]; qc = A.qz[in];
+    } else {
+        const int ip = off + a0 + i, in = off + a1 + i;
+        x = A.pcx[ip]; y = A.pcy[ip]; z = A.pcz[ip];
+        qa = A.qcx[in]; qb = A.qcy[in]; qc = A.qcz[in];
+    }
+}
+
+template <int PTS>
+IA_D bool pt_valid(const Arena &A, int off, int i) {
+    return PTS == PTS_FULL ? A.v[off + i] != 0 : true;
+}
+
+IA_D int bank_points(const BankDesc &B, const int *list_na) {
+    return B.pts == PTS_LIST ? list_na[B.a0] : B.n;
+}
+
+IA_D int bank_seeds(const BankDesc &B, int n) {
+    return B.seeds == SEED_EXPLICIT ? B.K : implicit_seed_count(n, B.max_starts);
+}
+
+// One candidate: iterative_seed_refine (search.h:28-56) or the gapless
+// offset start + refine_from_rt (threading.h:15-37, 415-430), then the
+// TM-score of search.h:358-362.
+template <int PTS>
+IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const SeedDesc *seeds,
+                        const uint8_t *masks, const int *lp, const int *ln, Hit &out) {
+    const PairInfo &pi = A.info[B.pair];
+    const int off = pi.off;
+    const int C = B.C;
+    const int k = j / C, c = j - k * C;
+    SeedDesc sd;
+    if (B.seeds == SEED_EXPLICIT) {
+        sd = seeds[B.seed_off + k];
+    } else {
+        int s0, len;
+        implicit_seed(n, B.max_starts, k, s0, len);
+        sd.a = s0;
+        sd.b = len;
+    }
+    double R[9], t[3];
+    KabschSeq ks;
+    ks.reset();
+    if (B.init == INIT_OFFSET) {
+        // gapless threading start: pred[i] <-> native[i-k] (threading.h:417-427)
+        const int kk = sd.a, N = pi.N;
+        for (int i = 0; i < N; i++) {
+            const int jj = i - kk;
+            if (jj < 0 || jj >= N || !A.pv[off + i] || !A.nv[off + jj]) continue;
+            ks.add_centroid(A.px[off + i], A.py[off + i], A.pz[off + i], A.qx[off + jj],
+                            A.qy[off + jj], A.qz[off + jj]);
+        }
+        if (ks.cnt < (double)B.min_ali) {
+            out.score = -INFINITY;
+            set_identity(out.R, out.t);
+            return;
+        }
+        ks.centre();
+        for (int i = 0; i < N; i++) {
+            const int jj = i - kk;
+            if (jj < 0 || jj >= N || !A.pv[off + i] || !A.nv[off + jj]) continue;
+            ks.add_cov(A.px[off + i], A.py[off + i], A.pz[off + i], A.qx[off + jj],
+                       A.qy[off + jj], A.qz[off + jj]);
+        }
+        ks.solve(R, t);
+    } else {
+        // seed & valid (search.h:34-35)
+        const bool win = sd.b > 0;
+        const uint8_t *m = masks + (win ? 0 : (long long)sd.a);
+        for (int i = 0; i < n; i++) {
+            const bool in = win ? (i >= sd.a && i < sd.a + sd.b) : (m[i] != 0);
+            if (!in || !pt_valid<PTS>(A, off, i)) continue;
+            double x, y, z, qa, qb, qc;
+            load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+            ks.add_centroid(x, y, z, qa, qb, qc);
+        }
+        if (ks.cnt < 3.0) {
+            set_identity(R, t);
+        } else {
+            ks.centre();
+            for (int i = 0; i < n; i++) {
+                const bool in = win ? (i >= sd.a && i < sd.a + sd.b) : (m[i] != 0);
+                if (!in || !pt_valid<PTS>(A, off, i)) continue;
+                double x, y, z, qa, qb, qc;
+                load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+                ks.add_cov(x, y, z, qa, qb, qc);
+            }
+            ks.solve(R, t);
+        }
+    }
+    // d0s-cutoff refinement; stop when the inlier count repeats.
+    const double cut = B.cut[c];
+    const double lim = cut * cut;
+    int last = -1;
+    for (int it = 0; it < B.iters; it++) {
+        ks.reset();
+        int cnt = 0;
+        for (int i = 0; i < n; i++) {
+            if (!pt_valid<PTS>(A, off, i)) continue;
+            double x, y, z, qa, qb, qc;
+            load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+            if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                cnt++;
+                ks.add_centroid(x, y, z, qa, qb, qc);
+            }
+        }
+        if (cnt >= 3) {
+            ks.centre();
+            for (int i = 0; i < n; i++) {
+                if (!pt_valid<PTS>(A, off, i)) continue;
+                double x, y, z, qa, qb, qc;
+                load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) ks.add_cov(x, y, z, qa, qb, qc);
+            }
+            ks.solve(R, t);
+        }
+        if (cnt == last) break;
+        last = cnt;
+    }
+    // TM-score over the bank's point set (core.h:134-166)
+    const double d0sq = fmax(pi.d0 * pi.d0, 1e-12), c8 = B.sd8 * B.sd8;
+    double acc = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (!pt_valid<PTS>(A, off, i)) continue;
+        double x, y, z, qa, qb, qc;
+        load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+        const double d2 = dist2_moved(R, t, x, y, z, qa, qb, qc);
+        if (B.use_d8 && !(d2 <= c8)) continue;
+        acc += tm_term(d2, d0sq);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) out.R[q] = R[q];
+    out.t[0] = t[0]; out.t[1] = t[1]; out.t[2] = t[2];
+    out.score = acc / fmax(pi.Lnorm, 1.0);
+}
+
+__global__ __launch_bounds__(256) void bank_kernel(Arena A, const BankDesc *banks,
+                                                   const int *blk_bank, const int *blk_first,
+                                                   const SeedDesc *seeds, const uint8_t *masks,
+                                                   const int *lists_p, const int *lists_n,
+                                                   const int *list_na, int list_cap,
+                                                   Hit *cands) {
+    const BankDesc &B = banks[blk_bank[blockIdx.x]];
+    const int j = blk_first[blockIdx.x] + (int)threadIdx.x;
+    if (j >= B.cand_cap) return;
+    Hit h;
+    const int n = bank_points(B, list_na);
+    const int K = bank_seeds(B, n);
+    if (j >= K * B.C || n <= 0) {
+        h.score = -INFINITY;
+        set_identity(h.R, h.t);
+    } else if (B.pts == PTS_FULL) {
+        run_candidate<PTS_FULL>(A, B, n, j, seeds, masks, nullptr, nullptr, h);
+    } else if (B.pts == PTS_LIST) {
+        const long long base = (long long)B.a0 * list_cap;
+        run_candidate<PTS_LIST>(A, B, n, j, seeds, masks, lists_p + base, lists_n + base, h);
+    } else {
+        run_candidate<PTS_CWIN>(A, B, n, j, seeds, masks, nullptr, nullptr, h);
+    }
+    cands[B.cand_off + j] = h;
+}
+
+// (score, index) first-wins maximum over a block.
+IA_D void block_argmax(double &s, long long &idx) {
+    __shared__ double ss[256];
+    __shared__ long long si[256];
+    const int t = threadIdx.x;
+    ss[t] = s;
+    si[t] = idx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            const double os = ss[t + w];
+            const long long oi = si[t + w];
+            if (oi >= 0 && (si[t] < 0 || os > ss[t] || (os == ss[t] && oi < si[t]))) {
+                ss[t] = os;
+                si[t] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    s = ss[0];
+    idx = si[0];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void bank_reduce_kernel(const BankDesc *banks,
+                                                          const int *list_na, const Hit *cands,
+                                                          Hit *out) {
+    const BankDesc &B = banks[blockIdx.x];
+    const int n = bank_points(B, list_na);
+    const int total = n > 0 ? bank_seeds(B, n) * B.C : 0;
+    const Hit *cb = cands + B.cand_off;
+    // serial early termination: stop after the first seed whose candidates
+    // lift the running best to >= et (search.h:83)
+    __shared__ int first;
+    if (threadIdx.x == 0) first = 0x7fffffff;
+    __syncthreads();
+    for (int j = threadIdx.x; j < total; j += 256)
+        if (cb[j].score >= B.et) atomicMin(&first, j);
+    __syncthreads();
+    const int limit = first == 0x7fffffff ? total : (first / B.C + 1) * B.C;
+    double bs = kNeg;
+    long long bi = -1;
+    for (int j = threadIdx.x; j < limit; j += 256) {
+        const double s = cb[j].score;
+        if (s > bs) { bs = s; bi = j; }
+    }
+    block_argmax(bs, bi);
+    if (threadIdx.x == 0) {
+        Hit h;
+        if (bi >= 0 && bs > kNeg) {
+            h = cb[bi];
+        } else {
+            set_identity(h.R, h.t);
+            h.score = kNeg;
+        }
+        out[B.out_idx] = h;
+    }
+}
+
+__global__ __launch_bounds__(256) void group_reduce_kernel(const long long *offs,
+                                                           const int *counts, const Hit *in,
+                                                           Hit *out) {
+    const Hit *g = in + offs[blockIdx.x];
+    const int cnt = counts[blockIdx.x];
+    double bs = kNeg;
+    long long bi = -1;
+    for (int j = threadIdx.x; j < cnt; j += 256) {
+        const double s = g[j].score;
+        if (s > bs) { bs = s; bi = j; }
+    }
+    block_argmax(bs, bi);
+    if (threadIdx.x == 0) {
+        Hit h;
+        if (bi >= 0 && bs > kNeg) {
+            h = g[bi];
+        } else {
+            // empty group: the reference's initial (-1e30, identity); a
+            // non-empty group of skipped entries stays skipped (-inf)
+            set_identity(h.R, h.t);
+            h.score = cnt > 0 ? -INFINITY : kNeg;
+        }
+        out[blockIdx.x] = h;
+    }
+}
+
+// Same-index TM-score without the d8 cut (core.h:152-166), one lane per item,
+// index-order sum.
+__global__ __launch_bounds__(256) void rescore_kernel(Arena A, const int *pair_of,
+                                                      const Hit *in, int n, Hit *out) {
+    const int it = blockIdx.x * 256 + threadIdx.x;
+    if (it >= n) return;
+    const PairInfo &pi = A.info[pair_of[it]];
+    Hit h = in[it];
+    if (h.score == -INFINITY) {  // skipped entry: nothing to rescore
+        out[it] = h;
+        return;
+    }
+    const double d0sq = fmax(pi.d0 * pi.d0, 1e-12);
+    double acc = 0.0;
+    for (int i = 0; i < pi.N; i++) {
+        const int o = pi.off + i;
+        if (!A.v[o]) continue;
+        acc += tm_term(dist2_moved(h.R, h.t, A.px[o], A.py[o], A.pz[o], A.qx[o], A.qy[o], A.qz[o]),
+                       d0sq);
+    }
+    h.score = acc / fmax(pi.Lnorm, 1.0);
+    out[it] = h;
+}
+
+// Top-k distance seed (indialign.cpp:156-161, seeds.h:85-98): the
+// k = min(max(3, floor(V/2)), N) valid residues with the smallest (d2, index).
+__global__ __launch_bounds__(256) void topk_kernel(Arena A, const int *pairs, const Hit *best,
+                                                   const long long *mask_off, uint8_t *masks) {
+    const PairInfo &pi = A.info[pairs[blockIdx.x]];
+    const Hit h = best[blockIdx.x];
+    uint8_t *m = masks + mask_off[blockIdx.x];
+    int k = pi.V / 2;  // (int)(V * 0.5)
+    if (k < 3) k = 3;
+    if (k > pi.N) k = pi.N;
+    for (int i = threadIdx.x; i < pi.N; i += 256) {
+        const int o = pi.off + i;
+        uint8_t sel = 0;
+        if (A.v[o]) {
+            const double di = dist2_moved(h.R, h.t, A.px[o], A.py[o], A.pz[o], A.qx[o], A.qy[o], A.qz[o]);
+            int rank = 0;
+            for (int j = 0; j < pi.N && rank < k; j++) {
+                const int p = pi.off + j;
+                if (!A.v[p] || j == i) continue;
+                const double dj = dist2_moved(h.R, h.t, A.px[p], A.py[p], A.pz[p], A.qx[p], A.qy[p], A.qz[p]);
+                rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+            }
+            sel = rank < k ? 1 : 0;
+        }
+        m[i] = sel;
+    }
+}
+
+// Weighted TM refinement, 4 iterations (core.h:183-237); one lane per pair,
+// index-order sums.
+__global__ __launch_bounds__(64) void weighted_kernel(Arena A, const int *pairs, const Hit *best,
+                                                      int n, Hit *out) {
+    const int it = blockIdx.x * 64 + threadIdx.x;
+    if (it >= n) return;
+    const PairInfo &pi = A.info[pairs[it]];
+    const int off = pi.off, N = pi.N;
+    Hit h = best[it];
+    const double d0sq = fmax(pi.d0 * pi.d0, 1e-12);
+    for (int iter = 0; iter < 4; iter++) {
+        // weights with the current transform, then weighted Kabsch
+        double cp[3] = {0, 0, 0}, cq[3] = {0, 0, 0}, ws = 0.0;
+        for (int i = 0; i < N; i++) {
+            const int o = off + i;
+            if (!A.v[o]) continue;
+            const double w = tm_term(dist2_moved(h.R, h.t, A.px[o], A.py[o], A.pz[o], A.qx[o], A.qy[o], A.qz[o]), d0sq);
+            if (w <= 0) continue;
+            cp[0] += w * A.px[o]; cp[1] += w * A.py[o]; cp[2] += w * A.pz[o];
+            cq[0] += w * A.qx[o]; cq[1] += w * A.qy[o]; cq[2] += w * A.qz[o];
+            ws += w;
+        }
+        if (ws < 1e-12) {
+            set_identity(h.R, h.t);
+            continue;
+        }
+        for (int d = 0; d < 3; d++) { cp[d] /= ws; cq[d] /= ws; }
+        double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < N; i++) {
+            const int o = off + i;
+            if (!A.v[o]) continue;
+            const double w = tm_term(dist2_moved(h.R, h.t, A.px[o], A.py[o], A.pz[o], A.qx[o], A.qy[o], A.qz[o]), d0sq);
+            if (w <= 0) continue;
+            const double a0 = A.px[o] - cp[0], a1 = A.py[o] - cp[1], a2 = A.pz[o] - cp[2];
+            const double b0 = A.qx[o] - cq[0], b1 = A.qy[o] - cq[1], b2 = A.qz[o] - cq[2];
+            H[0] += w * a0 * b0; H[1] += w * a0 * b1; H[2] += w * a0 * b2;
+            H[3] += w * a1 * b0; H[4] += w * a1 * b1; H[5] += w * a1 * b2;
+            H[6] += w * a2 * b0; H[7] += w * a2 * b1; H[8] += w * a2 * b2;
+        }
+        double R[9], t[3];
+        horn_rotation(H, R);
+        finish_t(cp, cq, R, t);
+        for (int q = 0; q < 9; q++) h.R[q] = R[q];
+        h.t[0] = t[0]; h.t[1] = t[1]; h.t[2] = t[2];
+    }
+    double acc = 0.0;
+    for (int i = 0; i < N; i++) {
+        const int o = off + i;
+        if (!A.v[o]) continue;
+        acc += tm_term(dist2_moved(h.R, h.t, A.px[o], A.py[o], A.pz[o], A.qx[o], A.qy[o], A.qz[o]), d0sq);
+    }
+    h.score = acc / fmax(pi.Lnorm, 1.0);
+    out[it] = h;
+}
+
+}  // namespace
+
+void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
+                 const int *blk_first, int n_blocks, const SeedDesc *seeds,
+                 const uint8_t *masks, const int *lists_p, const int *lists_n,
+                 const int *list_na, int list_cap, Hit *cands) {
+    if (n_blocks <= 0) return;
+    bank_kernel<<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks, lists_p,
+                                         lists_n, list_na, list_cap, cands);
+}
+
+void launch_bank_reduce(hipStream_t s, const BankDesc *banks, int n_banks, const int *list_na,
+                        const Hit *cands, Hit *out) {
+    if (n_banks <= 0) return;
+    bank_reduce_kernel<<<n_banks, 256, 0, s>>>(banks, list_na, cands, out);
+}
+
+void launch_group_reduce(hipStream_t s, const long long *offs, const int *counts, int n,
+                         const Hit *in, Hit *out) {
+    if (n <= 0) return;
+    group_reduce_kernel<<<n, 256, 0, s>>>(offs, counts, in, out);
+}
+
+void launch_rescore(hipStream_t s, const Arena &A, const int *pair_of, const Hit *in, int n,
+                    Hit *out) {
+    if (n <= 0) return;
+    rescore_kernel<<<(n + 255) / 256, 256, 0, s>>>(A, pair_of, in, n, out);
+}
+
+void launch_topk(hipStream_t s, const Arena &A, const int *pairs, const Hit *best, int n,
+                 const long long *mask_off, uint8_t *masks) {
+    if (n <= 0) return;
+    topk_kernel<<<n, 256, 0, s>>>(A, pairs, best, mask_off, masks);
+}
+
+void launch_weighted(hipStream_t s, const Arena &A, const int *pairs, const Hit *best, int n,
+                     Hit *out) {
+    if (n <= 0) return;
+    weighted_kernel<<<(n + 63) / 64, 64, 0, s>>>(A, pairs, best, n, out);
+}
+
+}  // namespace ia

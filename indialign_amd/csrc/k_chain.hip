@@ -1,0 +1,554 @@
+// DP-refine chains (align.h:54-145) on gfx950: one wavefront per chain.
+//
+// A chain is one gap-open branch of dp_refine_chunk: `iters` rounds of
+//   score matrix (fused, never stored) -> Needleman-Wunsch fill -> traceback
+//   -> Kabsch on the aligned rows -> detailed search -> keep the best.
+//
+// NW fill: rows are striped over the 64 lanes (lane l owns compact row
+// s0 + l of strip s0); the wavefront sweeps columns as a systolic pipeline,
+// lane l working on column t - l at step t, so every cell's up/diagonal
+// neighbours arrive from lane l-1 by a cross-lane shift and its left
+// neighbour is the lane's own previous result.  The last row of a strip is
+// handed to the next strip through an LDS row buffer.  Trace codes
+// (0 diag, 1 up, 2 left) are packed 16 cells per 32-bit word into LDS (or a
+// per-wavefront global slot for very long chains) and walked back by lane 0.
+//
+// Work distribution: a persistent grid pulls chain indices from an atomic
+// queue, so chains of very different cost balance across the CUs.
+#include "ia_kernels.h"
+
+#include <math.h>
+
+namespace ia {
+
+namespace {
+
+// d0 multipliers of the detailed search (seeds.h:12)
+IA_D double d0_mult(int g) {
+    switch (g) {
+        case 0: return 2.0;
+        case 1: return 1.5;
+        case 2: return 1.0;
+        case 3: return 0.75;
+        case 4: return 0.5;
+        default: return 0.25;
+    }
+}
+
+IA_D double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+IA_D double group8_sum(double v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    return v;
+}
+
+IA_D int group8_isum(int v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    return v;
+}
+
+// Wave-parallel two-pass Kabsch on n point pairs produced by get(q, ...).
+// Every lane ends with the same R, t (xor butterflies are order-symmetric).
+template <class Get>
+IA_D void wave_kabsch(int n, Get get, double *R, double *t) {
+    const int lane = threadIdx.x & 63;
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = lane; q < n; q += 64) {
+        double x, y, z, a, b, cc;
+        get(q, x, y, z, a, b, cc);
+        c[0] += x; c[1] += y; c[2] += z; c[3] += a; c[4] += b; c[5] += cc;
+    }
+#pragma unroll
+    for (int d = 0; d < 6; d++) c[d] = wave_sum(c[d]);
+    const double cnt = (double)n;
+    double cp[3] = {c[0] / cnt, c[1] / cnt, c[2] / cnt};
+    double cq[3] = {c[3] / cnt, c[4] / cnt, c[5] / cnt};
+    double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = lane; q < n; q += 64) {
+        double x, y, z, a, b, cc;
+        get(q, x, y, z, a, b, cc);
+        const double d0 = x - cp[0], d1 = y - cp[1], d2 = z - cp[2];
+        const double e0 = a - cq[0], e1 = b - cq[1], e2 = cc - cq[2];
+        h[0] += d0 * e0; h[1] += d0 * e1; h[2] += d0 * e2;
+        h[3] += d1 * e0; h[4] += d1 * e1; h[5] += d1 * e2;
+        h[6] += d2 * e0; h[7] += d2 * e1; h[8] += d2 * e2;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) h[k] = wave_sum(h[k]);
+    horn_rotation(h, R);
+    finish_t(cp, cq, R, t);
+}
+
+struct ChainLds {
+    unsigned *trace;  // plen x wpr words
+    double *rowH;     // nlen
+    signed char *rowT;
+    short *alP, *alN; // aligned compact indices, reverse order
+    int *misc;
+};
+
+// Needleman-Wunsch over the compact plen x nlen matrix with the score
+// computed on the fly; returns the aligned count (pairs in alP/alN, reverse
+// order).  nw.h:10-68 semantics.
+template <int SMODE>
+IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const double *t,
+                  double gap, double cut8, double d0sq_s, ChainLds &L, unsigned *trace) {
+    const int lane = threadIdx.x & 63;
+    const int plen = pi.plen, nlen = pi.nlen, off = pi.off;
+    const int wpr = (nlen + 15) >> 4;
+    for (int s0 = 0; s0 < plen; s0 += 64) {
+        const int a = s0 + lane;
+        const bool row_ok = a < plen;
+        const int rows = min(64, plen - s0);
+        double mx = 0, my = 0, mz = 0;
+        char sp = 0;
+        if (row_ok) {
+            if (SMODE != SM_SS) xform(R, t, A.pcx[off + a], A.pcy[off + a], A.pcz[off + a], mx, my, mz);
+            if (SMODE != SM_TM) sp = A.ssp[off + A.pmap[off + a]];
+        }
+        double Hleft = 0.0, Hdiag = 0.0, Hres = 0.0;
+        int Tleft = -1, Tres = -1;
+        unsigned tw = 0;
+        const int steps = nlen + rows - 1;
+        for (int st = 0; st < steps; st++) {
+            const int b = st - lane;
+            const bool act = row_ok && b >= 0 && b < nlen;
+            const double up_sh = __shfl_up(Hres, 1, 64);
+            const int tup_sh = __shfl_up(Tres, 1, 64);
+            double Hup;
+            int Tup;
+            if (lane == 0) {
+                if (s0 == 0 || !act) { Hup = 0.0; Tup = -1; }
+                else { Hup = L.rowH[b]; Tup = L.rowT[b]; }
+            } else {
+                Hup = up_sh;
+                Tup = tup_sh;
+            }
+            if (act) {
+                double s;
+                if (SMODE == SM_SS) {
+                    s = (sp == A.ssn[off + A.nmap[off + b]]) ? 1.0 : 0.0;
+                } else {
+                    const double dx = mx - A.qcx[off + b], dy = my - A.qcy[off + b],
+                                 dz = mz - A.qcz[off + b];
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    if (SMODE == SM_TM) {
+                        s = (d2 <= cut8) ? 1.0 / (1.0 + d2 / d0sq_s) : 0.0;
+                    } else {
+                        s = 1.0 / (1.0 + d2 / d0sq_s);
+                        if (sp == A.ssn[off + A.nmap[off + b]]) s += 0.5;
+                    }
+                }
+                const double hd = (b == 0) ? 0.0 : Hdiag;
+                const double hl = (b == 0) ? 0.0 : Hleft;
+                const int tl = (b == 0) ? -1 : Tleft;
+                const double diag = hd + s;
+                const double up = (Tup == 0) ? Hup + gap : Hup;
+                const double left = (tl == 0) ? hl + gap : hl;
+                double hn;
+                int tn;
+                if (diag >= up && diag >= left) { hn = diag; tn = 0; }
+                else if (left >= up) { hn = left; tn = 2; }
+                else { hn = up; tn = 1; }
+                tw |= (unsigned)tn << ((b & 15) * 2);
+                if ((b & 15) == 15 || b == nlen - 1) {
+                    trace[(long long)a * wpr + (b >> 4)] = tw;
+                    tw = 0;
+                }
+                if (lane == 63) { L.rowH[b] = hn; L.rowT[b] = (signed char)tn; }
+                Hleft = hn;
+                Tleft = tn;
+                Hres = hn;
+                Tres = tn;
+            }
+            Hdiag = Hup;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+    __builtin_amdgcn_wave_barrier();
+    // traceback (nw.h:42-68) by lane 0
+    if (lane == 0) {
+        int i = plen, j = nlen, k = 0;
+        while (i > 0 && j > 0) {
+            const unsigned w = trace[(long long)(i - 1) * wpr + ((j - 1) >> 4)];
+            const unsigned d = (w >> (((j - 1) & 15) * 2)) & 3u;
+            if (d == 0) {
+                L.alP[k] = (short)(i - 1);
+                L.alN[k] = (short)(j - 1);
+                k++;
+                i--;
+                j--;
+            } else if (d == 1) {
+                i--;
+            } else {
+                j--;
+            }
+        }
+        L.misc[0] = k;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    return L.misc[0];
+}
+
+// Lite detailed search (align.h:15-50 with use_frag = false): the whole
+// aligned set as the only seed, cutoffs d0s x {2,1.5,1,.75,.5,.25}, 8
+// refinement rounds, TM-score with the d8 cut.  Eight lanes per cutoff.
+IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int na,
+                      const double *nxR, const double *nxt, double sd8, Hit &det) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 3, s = lane & 7;
+    const bool gact = g < 6;
+    const int off = pi.off;
+    const double cut = pi.d0s * d0_mult(g);
+    const double lim = cut * cut;
+    double R[9], t[3];
+#pragma unroll
+    for (int q = 0; q < 9; q++) R[q] = nxR[q];
+    t[0] = nxt[0]; t[1] = nxt[1]; t[2] = nxt[2];
+    bool done = !gact;
+    int last = -1;
+    for (int it = 0; it < 8; it++) {
+        if (__all(done)) break;
+        double c[6] = {0, 0, 0, 0, 0, 0};
+        int cnt = 0;
+        if (!done) {
+            for (int q = s; q < na; q += 8) {
+                const int ap = L.alP[q], an = L.alN[q];
+                const double x = A.pcx[off + ap], y = A.pcy[off + ap], z = A.pcz[off + ap];
+                const double qa = A.qcx[off + an], qb = A.qcy[off + an], qc = A.qcz[off + an];
+                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                    cnt++;
+                    c[0] += x; c[1] += y; c[2] += z; c[3] += qa; c[4] += qb; c[5] += qc;
+                }
+            }
+        }
+        cnt = group8_isum(cnt);
+#pragma unroll
+        for (int d = 0; d < 6; d++) c[d] = group8_sum(c[d]);
+        double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const bool solve = !done && cnt >= 3;
+        double cp[3], cq[3];
+        const double fc = (double)cnt;
+        cp[0] = c[0] / fc; cp[1] = c[1] / fc; cp[2] = c[2] / fc;
+        cq[0] = c[3] / fc; cq[1] = c[4] / fc; cq[2] = c[5] / fc;
+        if (solve) {
+            for (int q = s; q < na; q += 8) {
+                const int ap = L.alP[q], an = L.alN[q];
+                const double x = A.pcx[off + ap], y = A.pcy[off + ap], z = A.pcz[off + ap];
+                const double qa = A.qcx[off + an], qb = A.qcy[off + an], qc = A.qcz[off + an];
+                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                    const double d0 = x - cp[0], d1 = y - cp[1], d2 = z - cp[2];
+                    const double e0 = qa - cq[0], e1 = qb - cq[1], e2 = qc - cq[2];
+                    h[0] += d0 * e0; h[1] += d0 * e1; h[2] += d0 * e2;
+                    h[3] += d1 * e0; h[4] += d1 * e1; h[5] += d1 * e2;
+                    h[6] += d2 * e0; h[7] += d2 * e1; h[8] += d2 * e2;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
+        if (solve) {
+            horn_rotation(h, R);
+            finish_t(cp, cq, R, t);
+        }
+        if (!done) {
+            if (cnt == last) done = true;
+            last = cnt;
+        }
+    }
+    // score with the d8 cut over the aligned set
+    const double d0sq = fmax(pi.d0 * pi.d0, 1e-12), c8 = sd8 * sd8;
+    double acc = 0.0;
+    if (gact) {
+        for (int q = s; q < na; q += 8) {
+            const int ap = L.alP[q], an = L.alN[q];
+            const double d2 = dist2_moved(R, t, A.pcx[off + ap], A.pcy[off + ap], A.pcz[off + ap],
+                                          A.qcx[off + an], A.qcy[off + an], A.qcz[off + an]);
+            if (d2 <= c8) acc += tm_term(d2, d0sq);
+        }
+    }
+    acc = group8_sum(acc);
+    const double score = acc / fmax(pi.Lnorm, 1.0);
+    // first maximum over the cutoffs in order (search.h:375)
+    double bs = kNeg;
+    int bg = -1;
+    for (int q = 0; q < 6; q++) {
+        const double sq = __shfl(score, q * 8, 64);
+        if (sq > bs) { bs = sq; bg = q; }
+    }
+    const int src = bg * 8;
+#pragma unroll
+    for (int q = 0; q < 9; q++) det.R[q] = __shfl(R[q], src, 64);
+#pragma unroll
+    for (int q = 0; q < 3; q++) det.t[q] = __shfl(t[q], src, 64);
+    det.score = bs;
+}
+
+IA_D int find_group(const ChainGroup *groups, int n_groups, long long task) {
+    int lo = 0, hi = n_groups - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (groups[mid].chain_off <= task) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int SMODE>
+IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int c,
+                    const Hit *xf, Hit *out, ListOut *lout, int *lists_p, int *lists_n,
+                    int *list_na, int list_cap, ChainLds &L, unsigned *trace) {
+    const int lane = threadIdx.x & 63;
+    const int off = pi.off;
+    const bool tm = SMODE == SM_TM;
+    const double gap = tm ? ((c & 1) ? 0.0 : -0.6) : G.gap_ss;
+    const int item = tm ? (c >> 1) : c;
+    const int g = tm ? (c & 1) : 0;
+    Hit *dst = out + G.out_off + (long long)item * G.out_stride + g;
+    double R[9], t[3];
+    bool skip = false;
+    if (G.init == CI_XFORM) {
+        const Hit &h = xf[G.xf_off + (long long)item * G.xf_stride + (G.xf_per_chain ? g : 0)];
+#pragma unroll
+        for (int q = 0; q < 9; q++) R[q] = h.R[q];
+        t[0] = h.t[0]; t[1] = h.t[1]; t[2] = h.t[2];
+    } else if (G.init == CI_FRAG) {
+        int L_ = 0;
+        while (L_ + 1 < G.n_lens && G.items_prefix[L_ + 1] <= item) L_++;
+        const int r = item - G.items_prefix[L_];
+        const int flen = G.lens[L_];
+        const int ps = (r / G.n_ns[L_]) * G.jp, ns = (r % G.n_ns[L_]) * G.jn;
+        wave_kabsch(flen,
+                    [&](int q, double &x, double &y, double &z, double &a, double &b, double &cc) {
+                        x = A.pcx[off + ps + q]; y = A.pcy[off + ps + q]; z = A.pcz[off + ps + q];
+                        a = A.qcx[off + ns + q]; b = A.qcy[off + ns + q]; cc = A.qcz[off + ns + q];
+                    },
+                    R, t);
+    } else {
+        const int k = G.k0 + item * G.kstep;
+        const int N = pi.N;
+        int cnt = 0;
+        for (int i = lane; i < N; i += 64) {
+            const int j = i - k;
+            cnt += (j >= 0 && j < N && A.pv[off + i] && A.nv[off + j]) ? 1 : 0;
+        }
+        for (int m = 32; m > 0; m >>= 1) cnt += __shfl_xor(cnt, m, 64);
+        if (k == 0 || cnt < 3) {
+            skip = true;
+        } else {
+            // compact the pairing in index order through a prefix count
+            double c6[6] = {0, 0, 0, 0, 0, 0};
+            for (int i = lane; i < N; i += 64) {
+                const int j = i - k;
+                if (j >= 0 && j < N && A.pv[off + i] && A.nv[off + j]) {
+                    c6[0] += A.px[off + i]; c6[1] += A.py[off + i]; c6[2] += A.pz[off + i];
+                    c6[3] += A.qx[off + j]; c6[4] += A.qy[off + j]; c6[5] += A.qz[off + j];
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < 6; d++) c6[d] = wave_sum(c6[d]);
+            const double fc = (double)cnt;
+            const double cp[3] = {c6[0] / fc, c6[1] / fc, c6[2] / fc};
+            const double cq[3] = {c6[3] / fc, c6[4] / fc, c6[5] / fc};
+            double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int i = lane; i < N; i += 64) {
+                const int j = i - k;
+                if (j >= 0 && j < N && A.pv[off + i] && A.nv[off + j]) {
+                    const double d0 = A.px[off + i] - cp[0], d1 = A.py[off + i] - cp[1], d2 = A.pz[off + i] - cp[2];
+                    const double e0 = A.qx[off + j] - cq[0], e1 = A.qy[off + j] - cq[1], e2 = A.qz[off + j] - cq[2];
+                    h[0] += d0 * e0; h[1] += d0 * e1; h[2] += d0 * e2;
+                    h[3] += d1 * e0; h[4] += d1 * e1; h[5] += d1 * e2;
+                    h[6] += d2 * e0; h[7] += d2 * e1; h[8] += d2 * e2;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 9; q++) h[q] = wave_sum(h[q]);
+            horn_rotation(h, R);
+            finish_t(cp, cq, R, t);
+        }
+    }
+    Hit best;
+    set_identity(best.R, best.t);
+    best.score = 0.0;  // align.h:61-64
+    if (skip) {
+        best.score = -INFINITY;
+        if (lane == 0) *dst = best;
+        return;
+    }
+    const double d0dp = fmax(pi.d0, 1.0);
+    double d0sq_s;
+    if (SMODE == SM_TM) d0sq_s = fmax(d0dp * d0dp, 1e-12);
+    else {
+        const double d01 = fmax(pi.d0 + 1.5, pi.d0s);  // ss.h:118-119
+        d0sq_s = d01 * d01;
+    }
+    const double cut8 = G.sd8 * G.sd8;
+    const double d0sq = fmax(pi.d0 * pi.d0, 1e-12);
+    for (int it = 0; it < G.iters; it++) {
+        const int na = nw_align<SMODE>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
+        if (G.mode == CM_LIST) {
+            const int slot = G.list_slot0 + c;
+            double nR[9], nt[3];
+            double sc = 0.0;
+            if (na >= 3) {
+                wave_kabsch(na,
+                            [&](int q, double &x, double &y, double &z, double &a, double &b, double &cc) {
+                                const int ap = L.alP[q], an = L.alN[q];
+                                x = A.pcx[off + ap]; y = A.pcy[off + ap]; z = A.pcz[off + ap];
+                                a = A.qcx[off + an]; b = A.qcy[off + an]; cc = A.qcz[off + an];
+                            },
+                            nR, nt);
+                double acc = 0.0;
+                for (int q = lane; q < na; q += 64) {
+                    const int ap = L.alP[q], an = L.alN[q];
+                    acc += tm_term(dist2_moved(nR, nt, A.pcx[off + ap], A.pcy[off + ap], A.pcz[off + ap],
+                                               A.qcx[off + an], A.qcy[off + an], A.qcz[off + an]), d0sq);
+                }
+                sc = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
+            } else {
+                set_identity(nR, nt);
+            }
+            // aligned pairs in increasing order, full indices
+            for (int q = lane; q < na; q += 64) {
+                const int rq = na - 1 - q;
+                lists_p[(long long)slot * list_cap + q] = A.pmap[off + L.alP[rq]];
+                lists_n[(long long)slot * list_cap + q] = A.nmap[off + L.alN[rq]];
+            }
+            if (lane == 0) {
+                list_na[slot] = na;
+                ListOut o;
+#pragma unroll
+                for (int q = 0; q < 9; q++) o.R[q] = nR[q];
+                o.t[0] = nt[0]; o.t[1] = nt[1]; o.t[2] = nt[2];
+                o.it_score = sc;
+                o.na = na;
+                o.pad = 0;
+                lout[slot] = o;
+            }
+            return;
+        }
+        if (na < 3) continue;  // align.h:106
+        double nR[9], nt[3];
+        wave_kabsch(na,
+                    [&](int q, double &x, double &y, double &z, double &a, double &b, double &cc) {
+                        const int ap = L.alP[q], an = L.alN[q];
+                        x = A.pcx[off + ap]; y = A.pcy[off + ap]; z = A.pcz[off + ap];
+                        a = A.qcx[off + an]; b = A.qcy[off + an]; cc = A.qcz[off + an];
+                    },
+                    nR, nt);
+        double acc = 0.0;
+        for (int q = lane; q < na; q += 64) {
+            const int ap = L.alP[q], an = L.alN[q];
+            acc += tm_term(dist2_moved(nR, nt, A.pcx[off + ap], A.pcy[off + ap], A.pcz[off + ap],
+                                       A.qcx[off + an], A.qcy[off + an], A.qcz[off + an]), d0sq);
+        }
+        const double it_score = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
+        Hit det;
+        lite_search(A, pi, L, na, nR, nt, G.sd8, det);
+        Hit step;
+        if (det.score > it_score) {
+            step = det;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; q++) step.R[q] = nR[q];
+            step.t[0] = nt[0]; step.t[1] = nt[1]; step.t[2] = nt[2];
+            step.score = it_score;
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) R[q] = step.R[q];
+        t[0] = step.t[0]; t[1] = step.t[1]; t[2] = step.t[2];
+        if (step.score > best.score) best = step;
+    }
+    if (lane == 0) *dst = best;
+}
+
+__global__ __launch_bounds__(64) void chain_kernel(Arena A, const ChainGroup *groups,
+                                                   int n_groups, long long n_chains,
+                                                   const Hit *xf, Hit *out, ListOut *lout,
+                                                   int *lists_p, int *lists_n, int *list_na,
+                                                   int list_cap, int max_plen, int max_nlen,
+                                                   int *queue, unsigned *trace_g,
+                                                   long long slot_words) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int wpr = (max_nlen + 15) >> 4;
+    const int mn = max_plen < max_nlen ? max_plen : max_nlen;
+    ChainLds L;
+    size_t o = 0;
+    L.rowH = (double *)(lds + o);
+    o += sizeof(double) * (size_t)max_nlen;
+    L.misc = (int *)(lds + o);
+    o += 16;
+    L.rowT = (signed char *)(lds + o);
+    o += ((size_t)max_nlen + 15) & ~(size_t)15;
+    L.alP = (short *)(lds + o);
+    o += ((size_t)mn * 2 + 15) & ~(size_t)15;
+    L.alN = (short *)(lds + o);
+    o += ((size_t)mn * 2 + 15) & ~(size_t)15;
+    L.trace = (unsigned *)(lds + o);
+    unsigned *trace = trace_g ? trace_g + (long long)blockIdx.x * slot_words : L.trace;
+    (void)wpr;
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        long long task = 0;
+        if (lane == 0) task = atomicAdd(queue, 1);
+        task = __shfl(task, 0, 64);
+        if (task >= n_chains) break;
+        const int gi = find_group(groups, n_groups, task);
+        const ChainGroup &G = groups[gi];
+        const int c = (int)(task - G.chain_off);
+        const PairInfo &pi = A.info[G.pair];
+        if (G.smat == SM_TM)
+            run_chain<SM_TM>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+        else if (G.smat == SM_SS)
+            run_chain<SM_SS>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+        else
+            run_chain<SM_IA4>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+}  // namespace
+
+int chain_blocks_per_cu(size_t lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel, 64, lds) != hipSuccess)
+        return 1;
+    return nb;
+}
+
+size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds) {
+    const int wpr = (max_nlen + 15) >> 4;
+    const int mn = max_plen < max_nlen ? max_plen : max_nlen;
+    size_t o = sizeof(double) * (size_t)max_nlen + 16;
+    o += ((size_t)max_nlen + 15) & ~(size_t)15;
+    o += 2 * (((size_t)mn * 2 + 15) & ~(size_t)15);
+    if (trace_in_lds) o += (size_t)max_plen * wpr * 4;
+    return o;
+}
+
+void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int n_groups,
+                   long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
+                   int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
+                   int *queue, unsigned *trace_scratch, long long trace_slot_words,
+                   int n_slots) {
+    if (n_chains <= 0) return;
+    const size_t lds = chain_lds_bytes(max_plen, max_nlen, trace_scratch == nullptr);
+    const int grid = n_slots;
+    hipLaunchKernelGGL(chain_kernel, dim3(grid), dim3(64), lds, s, A, groups, n_groups, n_chains,
+                       xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen, max_nlen,
+                       queue, trace_scratch, trace_slot_words);
+}
+
+}  // namespace ia

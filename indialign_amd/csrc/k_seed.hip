@@ -1,0 +1,221 @@
+// Seed construction on the device: anchor contact seeds (seeds.h:164-283)
+// and C1'-geometry secondary structure (ss.h:16-52).  One workgroup per pair.
+#include "ia_kernels.h"
+#include "stl_sort.h"
+
+#include <math.h>
+
+namespace ia {
+
+namespace {
+
+constexpr int kMaxSeen = 1024;  // existing seeds + anchors hashed per pair
+constexpr int kMaxAnchors = 5;  // indialign.cpp:74
+
+struct CountIdx {
+    int count, a;
+};
+struct ByCountDesc {
+    IA_HD bool operator()(const CountIdx &x, const CountIdx &y) const { return x.count > y.count; }
+};
+
+IA_D uint64_t fnv_step(uint64_t h, uint8_t b) {
+    h ^= b;
+    h *= 1099511628211ULL;
+    return h;
+}
+
+// Pair (i, j) counts as a consistent contact at radius r: the reference
+// visits j only inside the 27 grid cells of size r around i (cell index =
+// floor(x / r) per axis), then requires pd2 <= r2, nd2 <= r2 and
+// |pd - nd| <= tol (seeds.h:206-229).
+IA_D bool contact(const Arena &A, int off, int i, int j, double r, double r2, double tol) {
+    const double pix = A.px[off + i], piy = A.py[off + i], piz = A.pz[off + i];
+    const double pjx = A.px[off + j], pjy = A.py[off + j], pjz = A.pz[off + j];
+    const long long cx = (long long)floor(pix / r) - (long long)floor(pjx / r);
+    const long long cy = (long long)floor(piy / r) - (long long)floor(pjy / r);
+    const long long cz = (long long)floor(piz / r) - (long long)floor(pjz / r);
+    if (cx < -1 || cx > 1 || cy < -1 || cy > 1 || cz < -1 || cz > 1) return false;
+    const double pdx = pix - pjx, pdy = piy - pjy, pdz = piz - pjz;
+    const double pd2 = pdx * pdx + pdy * pdy + pdz * pdz;
+    if (pd2 > r2) return false;
+    const double ndx = A.qx[off + i] - A.qx[off + j], ndy = A.qy[off + i] - A.qy[off + j],
+                 ndz = A.qz[off + i] - A.qz[off + j];
+    const double nd2 = ndx * ndx + ndy * ndy + ndz * ndz;
+    if (nd2 > r2) return false;
+    return fabs(sqrt(pd2) - sqrt(nd2)) <= tol;
+}
+
+__global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
+                                                     const SeedDesc *frag, const int *frag_off,
+                                                     const int *frag_cnt, uint8_t *amask,
+                                                     const long long *aoff, int *acnt,
+                                                     int *scratch, const long long *soff) {
+    const int p = pairs[blockIdx.x];
+    const PairInfo &pi = A.info[p];
+    const int N = pi.N, off = pi.off, tid = threadIdx.x;
+    int *vidx = scratch + soff[blockIdx.x];          // V
+    int *counts = vidx + N;                          // V
+    CountIdx *cand = (CountIdx *)(counts + N);       // V (2 ints each)
+    uint8_t *out = amask + aoff[blockIdx.x];
+    __shared__ uint64_t seen[kMaxSeen];
+    __shared__ double radii[6];
+    __shared__ int nr, V, nseen, ncand, added, total, idx, dup;
+    __shared__ uint64_t hcur;
+    if (tid == 0) {
+        // radii d0s x {1, 1.5, 2} then d0 + {1, 2, 3} unless within 0.01
+        nr = 0;
+        radii[nr++] = pi.d0s;
+        radii[nr++] = pi.d0s * 1.5;
+        radii[nr++] = pi.d0s * 2.0;
+        for (int e = 1; e <= 3; e++) {
+            const double v = pi.d0 + (double)e;
+            bool d = false;
+            for (int j = 0; j < nr && !d; j++) d = fabs(radii[j] - v) < 0.01;
+            if (!d) radii[nr++] = v;
+        }
+        int c = 0;
+        for (int i = 0; i < N; i++)
+            if (A.v[off + i]) vidx[c++] = i;
+        V = c;
+        nseen = frag_cnt[blockIdx.x];
+        total = 0;
+    }
+    __syncthreads();
+    if (V < 3) {
+        if (tid == 0) acnt[blockIdx.x] = 0;
+        return;
+    }
+    // hashes of the existing masks: window [a, a+b) carries valid bytes
+    for (int k = tid; k < nseen; k += 256) {
+        const SeedDesc sd = frag[frag_off[blockIdx.x] + k];
+        uint64_t h = 1469598103934665603ULL;
+        for (int i = 0; i < N; i++)
+            h = fnv_step(h, (i >= sd.a && i < sd.a + sd.b) ? A.v[off + i] : (uint8_t)0);
+        seen[k] = h;
+    }
+    __syncthreads();
+    for (int ri = 0; ri < nr; ri++) {
+        const double r = radii[ri];
+        if (r <= 0) continue;
+        const double r2 = r * r, tol = fmax(0.5, r * 0.25);
+        for (int a = tid; a < V; a += 256) {
+            const int i = vidx[a];
+            int c = 1;
+            for (int b = 0; b < V; b++) {
+                const int j = vidx[b];
+                if (j != i && contact(A, off, i, j, r, r2, tol)) c++;
+            }
+            counts[a] = c;
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            int nc = 0;
+            for (int a = 0; a < V; a++)
+                if (counts[a] >= 3) cand[nc++] = CountIdx{counts[a], a};
+            ia_sort::sort(cand, cand + nc, ByCountDesc());
+            ncand = nc;
+            added = 0;
+            idx = 0;
+        }
+        __threadfence();
+        __syncthreads();
+        while (idx < ncand && added < kMaxAnchors) {
+            const int i = vidx[cand[idx].a];
+            uint8_t *m = out + (long long)total * N;
+            for (int j = tid; j < N; j += 256)
+                m[j] = (j == i || (A.v[off + j] && contact(A, off, i, j, r, r2, tol))) ? 1 : 0;
+            __threadfence();
+            __syncthreads();
+            if (tid == 0) {
+                uint64_t h = 1469598103934665603ULL;
+                for (int j = 0; j < N; j++) h = fnv_step(h, m[j]);
+                hcur = h;
+                dup = 0;
+            }
+            __syncthreads();
+            for (int k = tid; k < nseen; k += 256)
+                if (seen[k] == hcur) dup = 1;
+            __syncthreads();
+            if (tid == 0) {
+                // the neighbourhood always holds count >= 3 residues
+                if (!dup && nseen < kMaxSeen) {
+                    seen[nseen++] = hcur;
+                    added++;
+                    total++;
+                }
+                idx++;
+            }
+            __threadfence();
+            __syncthreads();
+        }
+    }
+    if (tid == 0) acnt[blockIdx.x] = total;
+}
+
+// Greedy stem detection (ss.h:16-52): for i ascending, pair i with the
+// largest j > i+3 such that (i,j) and (i+1,j-1) are 9-12 A apart and both
+// are still unpaired.  Side 0 = pred with pred-valid, 1 = native with
+// native-valid.  Writes 'L', '<', '>' by full index.
+__global__ __launch_bounds__(256) void ss_kernel(Arena A, char *ssp, char *ssn) {
+    const int p = blockIdx.x >> 1, side = blockIdx.x & 1;
+    const PairInfo &pi = A.info[p];
+    const int N = pi.N, off = pi.off, tid = threadIdx.x;
+    const double *X = side ? A.qx : A.px;
+    const double *Y = side ? A.qy : A.py;
+    const double *Z = side ? A.qz : A.pz;
+    const uint8_t *v = side ? A.nv : A.pv;
+    char *ss = (side ? ssn : ssp) + off;
+    __shared__ int best;
+    // mates are encoded in ss[] as: 'L' = free, '<' / '>' = paired
+    for (int i = tid; i < N; i += 256) ss[i] = 'L';
+    __threadfence();
+    __syncthreads();
+    for (int i = 0; i + 4 < N; i++) {
+        if (!v[off + i] || !v[off + i + 1]) continue;
+        if (ss[i] != 'L') continue;  // i already paired: no j can pass
+        int bj = -1;
+        const double xi = X[off + i], yi = Y[off + i], zi = Z[off + i];
+        const double xi1 = X[off + i + 1], yi1 = Y[off + i + 1], zi1 = Z[off + i + 1];
+        for (int j = N - 1 - tid; j > i + 3; j -= 256) {
+            if (!v[off + j] || !v[off + j - 1] || ss[j] != 'L') continue;
+            const double dx = xi - X[off + j], dy = yi - Y[off + j], dz = zi - Z[off + j];
+            const double a = dx * dx + dy * dy + dz * dz;
+            const double ex = xi1 - X[off + j - 1], ey = yi1 - Y[off + j - 1], ez = zi1 - Z[off + j - 1];
+            const double b = ex * ex + ey * ey + ez * ez;
+            if (a >= 81.0 && a <= 144.0 && b >= 81.0 && b <= 144.0) {
+                bj = j;
+                break;
+            }
+        }
+        if (tid == 0) best = -1;
+        __syncthreads();
+        if (bj >= 0) atomicMax(&best, bj);
+        __syncthreads();
+        if (tid == 0 && best >= 0) {
+            ss[i] = '<';
+            ss[best] = '>';
+        }
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+void launch_ss(hipStream_t s, const Arena &A, int n_pairs, char *ssp, char *ssn) {
+    if (n_pairs <= 0) return;
+    ss_kernel<<<2 * n_pairs, 256, 0, s>>>(A, ssp, ssn);
+}
+
+void launch_anchors(hipStream_t s, const Arena &A, const int *pairs, int n,
+                    const SeedDesc *frag_seeds, const int *frag_off, const int *frag_cnt,
+                    uint8_t *anchor_masks, const long long *anchor_off, int *anchor_cnt,
+                    int *scratch_i, const long long *scratch_off) {
+    if (n <= 0) return;
+    anchor_kernel<<<n, 256, 0, s>>>(A, pairs, frag_seeds, frag_off, frag_cnt, anchor_masks,
+                                    anchor_off, anchor_cnt, scratch_i, scratch_off);
+}
+
+}  // namespace ia
