@@ -88,7 +88,15 @@ int indialign_cuda_available(void);
 int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
                         NativePairResult *outputs, int serial_bank);
 
-/* Work and time of the last call on this thread's engine.  Fills up to n
+/* Resident batch: copy n_pairs inputs into device memory once
+ * (indialign_batch_load), then search them any number of times without
+ * host-to-device traffic (indialign_batch_run; outputs has n_pairs slots,
+ * same conventions as indialign_search_ex).  A later load or search call
+ * replaces the resident batch.  Return 0, or -2 on a device error. */
+int indialign_batch_load(int n_pairs, const NativePairInput *inputs);
+int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int serial_bank);
+
+/* Work and time of the last search.  Fills up to n
  * doubles:
  *   [0] wall ms  [1] candidates refined  [2] NW cells filled
  *   [3] chains run  [4] kernel launches

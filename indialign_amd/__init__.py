@@ -77,6 +77,11 @@ def load_library(path: str | None = None):
     lib.indialign_search_ex.restype = ctypes.c_int
     lib.indialign_cuda_available.argtypes = []
     lib.indialign_cuda_available.restype = ctypes.c_int
+    lib.indialign_batch_load.argtypes = [ctypes.c_int, ctypes.POINTER(NativePairInput)]
+    lib.indialign_batch_load.restype = ctypes.c_int
+    lib.indialign_batch_run.argtypes = [ctypes.POINTER(NativeConfig), ctypes.POINTER(NativePairResult),
+                                        ctypes.c_int]
+    lib.indialign_batch_run.restype = ctypes.c_int
     lib.indialign_last_stats.argtypes = [c_double_p, ctypes.c_int]
     lib.indialign_last_stats.restype = ctypes.c_int
     if path is None:
@@ -166,6 +171,30 @@ def tmscore_search_batch(pairs, cfg: NativeConfig | None = None, serial_bank: in
     if rc != 0:
         raise IndialignError(f"batch search returned {rc}")
     return _results(res, pk.n)
+
+
+_resident_n = 0
+
+
+def batch_load(pairs) -> int:
+    """Copy a batch into device memory once (indialign_batch_load)."""
+    global _resident_n
+    pk = _Packed([_as_tuple(p) for p in pairs])
+    rc = load_library().indialign_batch_load(pk.n, pk.inps)
+    if rc != 0:
+        raise IndialignError(f"indialign_batch_load returned {rc}")
+    _resident_n = pk.n
+    return pk.n
+
+
+def batch_run(cfg: NativeConfig | None = None, serial_bank: int = 1):
+    """Search the resident batch (indialign_batch_run); (scores, R, t)."""
+    cfg = cfg or default_config()
+    res = (NativePairResult * max(1, _resident_n))()
+    rc = load_library().indialign_batch_run(ctypes.byref(cfg), res, int(serial_bank))
+    if rc != 0:
+        raise IndialignError(f"indialign_batch_run returned {rc}")
+    return _results(res, _resident_n)
 
 
 def last_stats() -> dict:

@@ -117,6 +117,7 @@ struct HostPair {
     int N, V, plen, nlen;
     bool cross;
     PairInfo info;
+    std::vector<int> pre;        // prefix counts of valid residues
     std::vector<SeedDesc> frag;  // global + window seeds (seeds.h:41-82)
 };
 
@@ -142,7 +143,15 @@ class Engine {
     }
 
     int run(const NativePairInput *inp, int n, const NativeConfig *cfg, bool serial_bank,
-            NativePairResult *out);
+            NativePairResult *out) {
+        load(inp, n);
+        return search(cfg, serial_bank, out);
+    }
+    // Copy a batch into device memory (kept resident until the next load).
+    void load(const NativePairInput *inp, int n);
+    // Search the resident batch; out has one slot per loaded pair.
+    int search(const NativeConfig *cfg, bool serial_bank, NativePairResult *out);
+    int loaded() const { return n_loaded_; }
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
   private:
@@ -154,6 +163,8 @@ class Engine {
     std::vector<HostPair> hp_;
     std::vector<Hit> best_;
     std::vector<int> live_;  // pairs still in the pipeline (index into hp_)
+    int n_loaded_ = 0;
+    std::vector<signed char> status_;  // per loaded pair: 0 search, 1 Lnorm<=2, -1 N<=0
     Arena A_{};
     int list_cap_ = 1;
 
@@ -223,6 +234,7 @@ class Engine {
     void launched() { stats[4] += 1; }
 
     void prepare(const NativePairInput *inp, int n);
+    void enumerate_seeds();
     void stage_seed_bank();
     void stage_ss();
     void stage_gapless();
@@ -327,20 +339,7 @@ void Engine::prepare(const NativePairInput *inp, int n) {
         h.info.plen = pl;
         h.info.nlen = nl;
         h.info.V = V;
-        // global + window seeds over index positions, >= 3 valid (seeds.h:41-82)
-        if (V >= 3) h.frag.push_back(SeedDesc{0, N});
-        if (cfg_->use_fragment_search && V >= 4) {
-            int lens[9];
-            const int nlens = window_lengths(V, lens);
-            for (int L = 0; L < nlens; L++) {
-                const int cnt = window_start_count(N, lens[L], 50);
-                for (int q = 0; q < cnt; q++) {
-                    const int s0 = window_start(N, lens[L], 50, q);
-                    const int e = std::min(N, s0 + lens[L]);
-                    if (pre[e] - pre[s0] >= 3) h.frag.push_back(SeedDesc{s0, lens[L]});
-                }
-            }
-        }
+        h.pre = std::move(pre);
         info.push_back(h.info);
         hp_.push_back(std::move(h));
         off += N;
@@ -355,7 +354,27 @@ void Engine::prepare(const NativePairInput *inp, int n) {
     A_.ssn = b_ssn.get<char>(std::max(off, 1));
     A_.info = upload(b_info, info);
     IA_CHECK(hipStreamSynchronize(s_));
-    best_.assign(hp_.size(), identity_hit(kNeg));
+}
+
+// Global + window seeds over index positions with >= 3 valid residues
+// (seeds.h:41-82); depends on use_fragment_search, so done per search.
+void Engine::enumerate_seeds() {
+    for (HostPair &h : hp_) {
+        h.frag.clear();
+        const int N = h.N, V = h.V;
+        if (V >= 3) h.frag.push_back(SeedDesc{0, N});
+        if (!cfg_->use_fragment_search || V < 4) continue;
+        int lens[9];
+        const int nlens = window_lengths(V, lens);
+        for (int L = 0; L < nlens; L++) {
+            const int cnt = window_start_count(N, lens[L], 50);
+            for (int q = 0; q < cnt; q++) {
+                const int s0 = window_start(N, lens[L], 50, q);
+                const int e = std::min(N, s0 + lens[L]);
+                if (h.pre[e] - h.pre[s0] >= 3) h.frag.push_back(SeedDesc{s0, lens[L]});
+            }
+        }
+    }
 }
 
 std::vector<int> Engine::max_seed_table(int nmax, int ms) {
@@ -415,8 +434,12 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         mp = std::max(mp, h.plen);
         mn = std::max(mn, h.nlen);
         lc = std::max(lc, std::min(h.plen, h.nlen));
+        long long filled = g.n_chains;  // chains that run the NW fill
+        if (g.init == CI_OFFSET && g.k0 <= 0 && (-g.k0) % g.kstep == 0 &&
+            (-g.k0) / g.kstep < g.n_chains / 2)
+            filled -= 2;  // offset k = 0 is skipped (indialign.cpp:236)
         stats[3] += g.n_chains;
-        stats[2] += (double)h.plen * h.nlen * g.n_chains * g.iters;
+        stats[2] += (double)h.plen * h.nlen * (double)filled * g.iters;
     }
     if (total == 0) return 0;
     const ChainGroup *dg = upload(b_groups, groups);
@@ -1069,21 +1092,31 @@ void Engine::stage_weighted() {
     profile("weighted_refine", t0);
 }
 
-int Engine::run(const NativePairInput *inp, int n, const NativeConfig *cfg, bool serial_bank,
-                NativePairResult *out) {
+void Engine::load(const NativePairInput *inp, int n) {
+    n_loaded_ = n;
+    status_.assign(n, 0);
+    for (int k = 0; k < n; k++) {
+        if (inp[k].length <= 0) status_[k] = -1;
+        else if (inp[k].lnorm <= 2.0) status_[k] = 1;
+    }
+    prepare(inp, n);
+}
+
+int Engine::search(const NativeConfig *cfg, bool serial_bank, NativePairResult *out) {
     const auto t0 = std::chrono::steady_clock::now();
     for (double &x : stats) x = 0;
     cfg_ = cfg;
     serial_ = serial_bank;
     live_.clear();
+    best_.assign(hp_.size(), identity_hit(kNeg));
     // Lnorm <= 2 -> score 0, identity (indialign.cpp:47-51)
-    for (int k = 0; k < n; k++) {
-        if (inp[k].length <= 0 || inp[k].lnorm > 2.0) continue;
+    for (int k = 0; k < n_loaded_; k++) {
+        if (status_[k] != 1) continue;
         out[k].score = 0.0;
         set_identity(out[k].rotation, out[k].translation);
     }
-    prepare(inp, n);
     if (!hp_.empty()) {
+        enumerate_seeds();
         stage_seed_bank();
         stage_ss();
         stage_gapless();
@@ -1161,6 +1194,36 @@ int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const Native
                         NativePairResult *outputs, int serial_bank) {
     if (n_pairs <= 0) return 0;
     return run_search(n_pairs, inputs, cfg, outputs, serial_bank);
+}
+
+int indialign_batch_load(int n_pairs, const NativePairInput *inputs) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    try {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+            throw ia::DevError("no HIP device available");
+        if (!g_engine) g_engine.reset(new ia::Engine());
+        g_engine->load(inputs, n_pairs < 0 ? 0 : n_pairs);
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "indialign_amd: %s\n", e.what());
+        g_engine.reset();
+        return -2;
+    }
+}
+
+int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int serial_bank) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_engine) return -2;
+    try {
+        const int rc = g_engine->search(cfg, serial_bank != 0, outputs);
+        std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
+        return rc;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "indialign_amd: %s\n", e.what());
+        g_engine.reset();
+        return -2;
+    }
 }
 
 int indialign_cuda_available(void) {
