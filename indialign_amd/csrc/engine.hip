@@ -7,7 +7,7 @@
 // items, offsets), applies the stage gates and performs the strict-'>' merges
 // on per-pair results copied back after each stage.
 #include "ia_kernels.h"
-#include "../../include/indialign.h"
+#include "indialign.h"
 
 #include <algorithm>
 #include <chrono>
@@ -23,8 +23,6 @@
 
 namespace ia {
 
-size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds);
-int chain_blocks_per_cu(size_t lds);
 
 namespace {
 
@@ -447,8 +445,16 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     IA_CHECK(hipMemsetAsync(q, 0, sizeof(int), s_));
     const int wpr = (mn + 15) / 16;
     const size_t trace_bytes = (size_t)mp * wpr * 4;
-    const bool in_lds = trace_bytes <= 96 * 1024;
-    const size_t lds = chain_lds_bytes(mp, mn, in_lds);
+    // LDS layout: trace and staged columns in LDS when they fit, else the
+    // trace in a per-wavefront global slot, else the columns from global too.
+    constexpr size_t kLdsMax = 160 * 1024;
+    const bool in_lds = trace_bytes <= 96 * 1024 && chain_lds_bytes(mp, mn, true, true) <= kLdsMax;
+    // stage the columns only when that costs no resident wavefront
+    const size_t lds_c = chain_lds_bytes(mp, mn, in_lds, true);
+    const size_t lds_g = chain_lds_bytes(mp, mn, in_lds, false);
+    const bool cols_lds = lds_c <= kLdsMax && kLdsMax / lds_c >= std::min<size_t>(8, kLdsMax / lds_g);
+    const size_t lds = cols_lds ? lds_c : lds_g;
+    if (lds > kLdsMax) throw DevError("chain too long for one workgroup's LDS");
     int per_cu = std::max(1, chain_blocks_per_cu(lds));
     per_cu = std::min(per_cu, 16);
     long long slots = (long long)n_cu_ * per_cu;
@@ -464,7 +470,8 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     IA_CHECK(hipEventRecord(e0_, s_));
     launch_chains(s_, A_, dg, (int)groups.size(), total, xf_dev, out_dev,
                   b_lout.get<ListOut>(1), b_lists_p.get<int>(1), b_lists_n.get<int>(1),
-                  b_list_na.get<int>(1), list_cap_, mp, mn, q, trace, slot_words, (int)slots);
+                  b_list_na.get<int>(1), list_cap_, mp, mn, q, trace, slot_words, (int)slots,
+                  cols_lds);
     launched();
     IA_CHECK(hipGetLastError());
     IA_CHECK(hipEventRecord(e1_, s_));

@@ -101,6 +101,8 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
-                   int n_slots);
+                   int n_slots, bool cols_in_lds);
+size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds);
+int chain_blocks_per_cu(size_t lds);
 
 }  // namespace ia

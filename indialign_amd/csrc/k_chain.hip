@@ -88,92 +88,169 @@ IA_D void wave_kabsch(int n, Get get, double *R, double *t) {
 }
 
 struct ChainLds {
-    unsigned *trace;  // plen x wpr words
+    unsigned *trace;  // plen x wpr words (when the trace fits in LDS)
+    double *colx, *coly, *colz;  // the chain's compact native coordinates
+    char *cols;                  // compact native secondary structure
     double *rowH;     // nlen
     signed char *rowT;
     short *alP, *alN; // aligned compact indices, reverse order
     int *misc;
 };
 
-// Needleman-Wunsch over the compact plen x nlen matrix with the score
-// computed on the fly; returns the aligned count (pairs in alP/alN, reverse
-// order).  nw.h:10-68 semantics.
-template <int SMODE>
-IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const double *t,
-                  double gap, double cut8, double d0sq_s, ChainLds &L, unsigned *trace) {
+// Cross-lane shift by one lane (DPP wave_shr:1, a VALU move; no LDS
+// round trip).  Lane 0 keeps its own value.
+IA_D int shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+IA_D double shr1(double v) {
+    const int lo = shr1(__double2loint(v)), hi = shr1(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Rotation by one lane (DPP wave_ror:1): lane l receives lane l-1, lane 0
+// receives lane 63.
+IA_D int ror1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x13C, 0xf, 0xf, false); }
+IA_D double ror1(double v) {
+    const int lo = ror1(__double2loint(v)), hi = ror1(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Needleman-Wunsch fill (nw.h:10-38) over the compact plen x nlen matrix,
+// score computed on the fly.
+//
+// The wavefront sweeps super-strips of 64*K rows.  Lane l owns rows
+// s0 + l + 64 j (slot j = 0..K-1) and, at step st, works on column
+// st - l - 64 j of each of them: the K cells of a lane are independent, so
+// their dependency chains overlap.  Up/diagonal operands come from lane l-1
+// by a lane rotation; lane 0 of slot j receives lane 63 of slot j-1 the same
+// way, and slot 0's lane 0 reads the previous super-strip's last row from
+// LDS.  Each cell's value H is max(diag, up, left) (ties only decide the
+// trace code), and G = H + gap_open when the cell was entered diagonally,
+// else H, is what its down and right neighbours add.
+template <int SMODE, class TR, bool CL, int K>
+IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const double *t,
+                  double gap, double cut8, double d0sq_s, ChainLds &L, TR *trace) {
     const int lane = threadIdx.x & 63;
     const int plen = pi.plen, nlen = pi.nlen, off = pi.off;
     const int wpr = (nlen + 15) >> 4;
-    for (int s0 = 0; s0 < plen; s0 += 64) {
-        const int a = s0 + lane;
-        const bool row_ok = a < plen;
-        const int rows = min(64, plen - s0);
-        double mx = 0, my = 0, mz = 0;
-        char sp = 0;
-        if (row_ok) {
-            if (SMODE != SM_SS) xform(R, t, A.pcx[off + a], A.pcy[off + a], A.pcz[off + a], mx, my, mz);
-            if (SMODE != SM_TM) sp = A.ssp[off + A.pmap[off + a]];
-        }
-        double Hleft = 0.0, Hdiag = 0.0, Hres = 0.0;
-        int Tleft = -1, Tres = -1;
-        unsigned tw = 0;
-        const int steps = nlen + rows - 1;
-        for (int st = 0; st < steps; st++) {
-            const int b = st - lane;
-            const bool act = row_ok && b >= 0 && b < nlen;
-            const double up_sh = __shfl_up(Hres, 1, 64);
-            const int tup_sh = __shfl_up(Tres, 1, 64);
-            double Hup;
-            int Tup;
-            if (lane == 0) {
-                if (s0 == 0 || !act) { Hup = 0.0; Tup = -1; }
-                else { Hup = L.rowH[b]; Tup = L.rowT[b]; }
-            } else {
-                Hup = up_sh;
-                Tup = tup_sh;
+    auto colx = [&](int b) { return CL ? L.colx[b] : A.qcx[off + b]; };
+    auto coly = [&](int b) { return CL ? L.coly[b] : A.qcy[off + b]; };
+    auto colz = [&](int b) { return CL ? L.colz[b] : A.qcz[off + b]; };
+    auto cols = [&](int b) { return CL ? L.cols[b] : A.ssn[off + A.nmap[off + b]]; };
+    for (int s0 = 0; s0 < plen; s0 += 64 * K) {
+        const int rows = min(64 * K, plen - s0);
+        const bool first = s0 == 0;
+        const bool has_next = s0 + 64 * K < plen;
+        double mx[K], my[K], mz[K], H[K], G[K], D[K], cx[K], cy[K], cz[K];
+        char sp[K], cs[K];
+        bool rok[K];
+        unsigned tw[K];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int a = s0 + 64 * j + lane;
+            rok[j] = a < plen;
+            mx[j] = my[j] = mz[j] = 0.0;
+            sp[j] = 0;
+            if (rok[j]) {
+                if (SMODE != SM_SS) xform(R, t, A.pcx[off + a], A.pcy[off + a], A.pcz[off + a], mx[j], my[j], mz[j]);
+                if (SMODE != SM_TM) sp[j] = A.ssp[off + A.pmap[off + a]];
             }
-            if (act) {
+            H[j] = G[j] = D[j] = 0.0;
+            tw[j] = 0;
+            const int b = min(max(-64 * j - lane, 0), nlen - 1);
+            cx[j] = cy[j] = cz[j] = 0.0;
+            cs[j] = 0;
+            if (SMODE != SM_SS) { cx[j] = colx(b); cy[j] = coly(b); cz[j] = colz(b); }
+            if (SMODE != SM_TM) cs[j] = cols(b);
+        }
+        const int steps = nlen + rows - 1;
+        double rH = first ? 0.0 : L.rowH[0];
+        int rT = first ? -1 : (int)L.rowT[0];
+        for (int st = 0; st < steps; st++) {
+            const int rn = min(st + 1, nlen - 1);
+            const double rHn = first ? 0.0 : L.rowH[rn];
+            const int rTn = first ? -1 : (int)L.rowT[rn];
+            double XH[K], XG[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                XH[j] = ror1(H[j]);
+                XG[j] = ror1(G[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const int b = st - 64 * j - lane;
+                const bool act = rok[j] && b >= 0 && b < nlen;
+                // next step's column operands
+                const int bn = min(max(b + 1, 0), nlen - 1);
+                double nx = 0, ny = 0, nz = 0;
+                char ns = 0;
+                if (SMODE != SM_SS) { nx = colx(bn); ny = coly(bn); nz = colz(bn); }
+                if (SMODE != SM_TM) ns = cols(bn);
                 double s;
                 if (SMODE == SM_SS) {
-                    s = (sp == A.ssn[off + A.nmap[off + b]]) ? 1.0 : 0.0;
+                    s = (sp[j] == cs[j]) ? 1.0 : 0.0;
                 } else {
-                    const double dx = mx - A.qcx[off + b], dy = my - A.qcy[off + b],
-                                 dz = mz - A.qcz[off + b];
+                    const double dx = mx[j] - cx[j], dy = my[j] - cy[j], dz = mz[j] - cz[j];
                     const double d2 = dx * dx + dy * dy + dz * dz;
                     if (SMODE == SM_TM) {
                         s = (d2 <= cut8) ? 1.0 / (1.0 + d2 / d0sq_s) : 0.0;
                     } else {
                         s = 1.0 / (1.0 + d2 / d0sq_s);
-                        if (sp == A.ssn[off + A.nmap[off + b]]) s += 0.5;
+                        if (sp[j] == cs[j]) s += 0.5;
                     }
                 }
-                const double hd = (b == 0) ? 0.0 : Hdiag;
-                const double hl = (b == 0) ? 0.0 : Hleft;
-                const int tl = (b == 0) ? -1 : Tleft;
-                const double diag = hd + s;
-                const double up = (Tup == 0) ? Hup + gap : Hup;
-                const double left = (tl == 0) ? hl + gap : hl;
-                double hn;
-                int tn;
-                if (diag >= up && diag >= left) { hn = diag; tn = 0; }
-                else if (left >= up) { hn = left; tn = 2; }
-                else { hn = up; tn = 1; }
-                tw |= (unsigned)tn << ((b & 15) * 2);
-                if ((b & 15) == 15 || b == nlen - 1) {
-                    trace[(long long)a * wpr + (b >> 4)] = tw;
-                    tw = 0;
+                // up operands (raw H for the next diagonal, G for "up")
+                double hu, gu;
+                if (j == 0) {
+                    const bool bound = first || b >= nlen;
+                    const double h0 = bound ? 0.0 : rH;
+                    const double g0 = bound ? 0.0 : ((rT == 0) ? rH + gap : rH);
+                    hu = lane == 0 ? h0 : XH[0];
+                    gu = lane == 0 ? g0 : XG[0];
+                } else {
+                    hu = lane == 0 ? XH[j > 0 ? j - 1 : 0] : XH[j];
+                    gu = lane == 0 ? XG[j > 0 ? j - 1 : 0] : XG[j];
                 }
-                if (lane == 63) { L.rowH[b] = hn; L.rowT[b] = (signed char)tn; }
-                Hleft = hn;
-                Tleft = tn;
-                Hres = hn;
-                Tres = tn;
+                const bool edge = b == 0;
+                const double diag = (edge ? 0.0 : D[j]) + s;
+                const double left = edge ? 0.0 : G[j];
+                const double hm = fmax(diag, fmax(gu, left));
+                const int tn = (diag >= gu && diag >= left) ? 0 : (left >= gu ? 2 : 1);
+                const double gn = (tn == 0) ? hm + gap : hm;
+                if (act) {
+                    tw[j] |= (unsigned)tn << ((b & 15) * 2);
+                    if ((b & 15) == 15 || b == nlen - 1) {
+                        trace[(long long)(s0 + 64 * j + lane) * wpr + (b >> 4)] = tw[j];
+                        tw[j] = 0;
+                    }
+                    if (j == K - 1 && lane == 63 && has_next) {
+                        L.rowH[b] = hm;
+                        L.rowT[b] = (signed char)tn;
+                    }
+                }
+                D[j] = hu;
+                H[j] = hm;
+                G[j] = gn;
+                cx[j] = nx; cy[j] = ny; cz[j] = nz; cs[j] = ns;
             }
-            Hdiag = Hup;
+            rH = rHn;
+            rT = rTn;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
     }
+}
+
+// Needleman-Wunsch fill + traceback; returns the aligned count (pairs in
+// alP/alN, reverse order).  nw.h:10-68 semantics.
+template <int SMODE, class TR, bool CL>
+IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const double *t,
+                  double gap, double cut8, double d0sq_s, ChainLds &L, TR *trace) {
+    const int lane = threadIdx.x & 63;
+    const int plen = pi.plen, nlen = pi.nlen;
+    const int wpr = (nlen + 15) >> 4;
+    if (plen <= 64)
+        nw_fill<SMODE, TR, CL, 1>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
+    else
+        nw_fill<SMODE, TR, CL, 2>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
     __builtin_amdgcn_wave_barrier();
     // traceback (nw.h:42-68) by lane 0
@@ -305,12 +382,22 @@ IA_D int find_group(const ChainGroup *groups, int n_groups, long long task) {
     return lo;
 }
 
-template <int SMODE>
+template <int SMODE, class TR, bool CL>
 IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int c,
                     const Hit *xf, Hit *out, ListOut *lout, int *lists_p, int *lists_n,
-                    int *list_na, int list_cap, ChainLds &L, unsigned *trace) {
+                    int *list_na, int list_cap, ChainLds &L, TR *trace) {
     const int lane = threadIdx.x & 63;
     const int off = pi.off;
+    for (int b = lane; CL && b < pi.nlen; b += 64) {
+        if (SMODE != SM_SS) {
+            L.colx[b] = A.qcx[off + b];
+            L.coly[b] = A.qcy[off + b];
+            L.colz[b] = A.qcz[off + b];
+        }
+        if (SMODE != SM_TM) L.cols[b] = A.ssn[off + A.nmap[off + b]];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
     const bool tm = SMODE == SM_TM;
     const double gap = tm ? ((c & 1) ? 0.0 : -0.6) : G.gap_ss;
     const int item = tm ? (c >> 1) : c;
@@ -396,7 +483,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     const double cut8 = G.sd8 * G.sd8;
     const double d0sq = fmax(pi.d0 * pi.d0, 1e-12);
     for (int it = 0; it < G.iters; it++) {
-        const int na = nw_align<SMODE>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
+        const int na = nw_align<SMODE, TR, CL>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
         if (G.mode == CM_LIST) {
             const int slot = G.list_slot0 + c;
             double nR[9], nt[3];
@@ -473,31 +560,10 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     if (lane == 0) *dst = best;
 }
 
-__global__ __launch_bounds__(64) void chain_kernel(Arena A, const ChainGroup *groups,
-                                                   int n_groups, long long n_chains,
-                                                   const Hit *xf, Hit *out, ListOut *lout,
-                                                   int *lists_p, int *lists_n, int *list_na,
-                                                   int list_cap, int max_plen, int max_nlen,
-                                                   int *queue, unsigned *trace_g,
-                                                   long long slot_words) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    const int wpr = (max_nlen + 15) >> 4;
-    const int mn = max_plen < max_nlen ? max_plen : max_nlen;
-    ChainLds L;
-    size_t o = 0;
-    L.rowH = (double *)(lds + o);
-    o += sizeof(double) * (size_t)max_nlen;
-    L.misc = (int *)(lds + o);
-    o += 16;
-    L.rowT = (signed char *)(lds + o);
-    o += ((size_t)max_nlen + 15) & ~(size_t)15;
-    L.alP = (short *)(lds + o);
-    o += ((size_t)mn * 2 + 15) & ~(size_t)15;
-    L.alN = (short *)(lds + o);
-    o += ((size_t)mn * 2 + 15) & ~(size_t)15;
-    L.trace = (unsigned *)(lds + o);
-    unsigned *trace = trace_g ? trace_g + (long long)blockIdx.x * slot_words : L.trace;
-    (void)wpr;
+template <class TR, bool CL>
+IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, long long n_chains,
+                     const Hit *xf, Hit *out, ListOut *lout, int *lists_p, int *lists_n,
+                     int *list_na, int list_cap, int *queue, ChainLds &L, TR *trace) {
     const int lane = threadIdx.x & 63;
     while (true) {
         long long task = 0;
@@ -509,13 +575,74 @@ __global__ __launch_bounds__(64) void chain_kernel(Arena A, const ChainGroup *gr
         const int c = (int)(task - G.chain_off);
         const PairInfo &pi = A.info[G.pair];
         if (G.smat == SM_TM)
-            run_chain<SM_TM>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+            run_chain<SM_TM, TR, CL>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         else if (G.smat == SM_SS)
-            run_chain<SM_SS>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+            run_chain<SM_SS, TR, CL>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         else
-            run_chain<SM_IA4>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+            run_chain<SM_IA4, TR, CL>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// LDS carve-out of one wavefront (shared with chain_lds_bytes).
+struct LdsPlan {
+    size_t rowH, misc, rowT, alP, alN, colx, coly, colz, cols, trace, total;
+    IA_HD LdsPlan(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds) {
+        const size_t mn = (size_t)(max_plen < max_nlen ? max_plen : max_nlen);
+        const size_t n = (size_t)max_nlen, nc = cols_in_lds ? n : 0;
+        size_t o = 0;
+        rowH = o; o += 8 * n;
+        colx = o; o += 8 * nc;
+        coly = o; o += 8 * nc;
+        colz = o; o += 8 * nc;
+        misc = o; o += 16;
+        rowT = o; o += (n + 15) & ~(size_t)15;
+        cols = o; o += (nc + 15) & ~(size_t)15;
+        alP = o; o += (2 * mn + 15) & ~(size_t)15;
+        alN = o; o += (2 * mn + 15) & ~(size_t)15;
+        trace = o;
+        if (trace_in_lds) o += (size_t)max_plen * (size_t)((max_nlen + 15) >> 4) * 4;
+        total = o;
+    }
+};
+
+__global__ __launch_bounds__(64) void chain_kernel(Arena A, const ChainGroup *groups,
+                                                   int n_groups, long long n_chains,
+                                                   const Hit *xf, Hit *out, ListOut *lout,
+                                                   int *lists_p, int *lists_n, int *list_na,
+                                                   int list_cap, int max_plen, int max_nlen,
+                                                   int *queue, unsigned *trace_g,
+                                                   long long slot_words, int cols_in_lds) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const LdsPlan P(max_plen, max_nlen, trace_g == nullptr, cols_in_lds != 0);
+    ChainLds L;
+    L.rowH = (double *)(lds + P.rowH);
+    L.colx = (double *)(lds + P.colx);
+    L.coly = (double *)(lds + P.coly);
+    L.colz = (double *)(lds + P.colz);
+    L.misc = (int *)(lds + P.misc);
+    L.rowT = (signed char *)(lds + P.rowT);
+    L.cols = (char *)(lds + P.cols);
+    L.alP = (short *)(lds + P.alP);
+    L.alN = (short *)(lds + P.alN);
+    L.trace = (unsigned *)(lds + P.trace);
+    if (trace_g) {
+        unsigned *tg = trace_g + (long long)blockIdx.x * slot_words;
+        if (cols_in_lds)
+            chain_loop<unsigned, true>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+                                       lists_n, list_na, list_cap, queue, L, tg);
+        else
+            chain_loop<unsigned, false>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+                                        lists_n, list_na, list_cap, queue, L, tg);
+    } else {
+        typedef __attribute__((address_space(3))) unsigned lds_u32;
+        if (cols_in_lds)
+            chain_loop<lds_u32, true>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+                                      lists_n, list_na, list_cap, queue, L, (lds_u32 *)L.trace);
+        else
+            chain_loop<lds_u32, false>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+                                       lists_n, list_na, list_cap, queue, L, (lds_u32 *)L.trace);
     }
 }
 
@@ -528,27 +655,21 @@ int chain_blocks_per_cu(size_t lds) {
     return nb;
 }
 
-size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds) {
-    const int wpr = (max_nlen + 15) >> 4;
-    const int mn = max_plen < max_nlen ? max_plen : max_nlen;
-    size_t o = sizeof(double) * (size_t)max_nlen + 16;
-    o += ((size_t)max_nlen + 15) & ~(size_t)15;
-    o += 2 * (((size_t)mn * 2 + 15) & ~(size_t)15);
-    if (trace_in_lds) o += (size_t)max_plen * wpr * 4;
-    return o;
+size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds) {
+    return LdsPlan(max_plen, max_nlen, trace_in_lds, cols_in_lds).total;
 }
 
 void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int n_groups,
                    long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
-                   int n_slots) {
+                   int n_slots, bool cols_in_lds) {
     if (n_chains <= 0) return;
-    const size_t lds = chain_lds_bytes(max_plen, max_nlen, trace_scratch == nullptr);
+    const size_t lds = chain_lds_bytes(max_plen, max_nlen, trace_scratch == nullptr, cols_in_lds);
     const int grid = n_slots;
     hipLaunchKernelGGL(chain_kernel, dim3(grid), dim3(64), lds, s, A, groups, n_groups, n_chains,
                        xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen, max_nlen,
-                       queue, trace_scratch, trace_slot_words);
+                       queue, trace_scratch, trace_slot_words, (int)cols_in_lds);
 }
 
 }  // namespace ia
