@@ -57,6 +57,16 @@ def cpu_baseline(pairs, sample):
                       f"reference batch entry, {dt:.1f} s"}
 
 
+def measured_traffic():
+    """HBM bytes per chain-kernel launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/chain_kernel_short512.md)."""
+    p = os.path.join(ROOT, "profiles", "chain_kernel_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +75,19 @@ def main():
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy kernel of this size (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(pairs are independent; see DESIGN.md for how the batch would shard)", file=sys.stderr)
@@ -122,7 +144,7 @@ def main():
         "roofline": {"kernel": "chain_kernel (fused TM score matrix + NW + traceback + Kabsch + lite search)",
                      "bound": "valu", "achieved": round(achieved, 3), "peak": FP64_VALU_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_PEAK_TFLOPS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic()},
         "mean_score": round(float(scores.mean()), 6),
     }
     if not args.no_cpu:
