@@ -81,12 +81,17 @@ int indialign_cuda_available(void);
 
 /* ---- extensions (no counterpart in the reference) ---------------------- */
 
-/* Batch search with an explicit seed-bank convention: serial_bank = 1 applies
- * early_term_score to the main seed bank (the reference's batch / one-thread
- * semantics), 0 never stops early (its multithreaded single-pair
- * semantics).  Same return codes as the batch entry. */
+/* Batch search with an explicit seed-bank convention and method.
+ * serial_bank = 1 applies early_term_score to the main seed bank (the
+ * reference's batch / one-thread semantics), 0 never stops early (its
+ * multithreaded single-pair semantics).
+ * method = 0: plain FP64, the reference's arithmetic (results identical to
+ * the reference CPU path up to its own rounding); method = 1: fast (FP32
+ * Needleman-Wunsch fill, closed-form FP64 rotations; every reported score is
+ * still an FP64 TM-score of the returned transform).  Same return codes as
+ * the batch entry, -3 for an unknown method. */
 int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
-                        NativePairResult *outputs, int serial_bank);
+                        NativePairResult *outputs, int serial_bank, int method);
 
 /* Resident batch: copy n_pairs inputs into device memory once
  * (indialign_batch_load), then search them any number of times without
@@ -94,7 +99,8 @@ int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const Native
  * same conventions as indialign_search_ex).  A later load or search call
  * replaces the resident batch.  Return 0, or -2 on a device error. */
 int indialign_batch_load(int n_pairs, const NativePairInput *inputs);
-int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int serial_bank);
+int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int serial_bank,
+                        int method);
 
 /* Work and time of the last search.  Fills up to n
  * doubles:

@@ -73,14 +73,15 @@ def load_library(path: str | None = None):
     lib.indialign_tmscore_search_batch.restype = ctypes.c_int
     lib.indialign_search_ex.argtypes = [ctypes.c_int, ctypes.POINTER(NativePairInput),
                                         ctypes.POINTER(NativeConfig),
-                                        ctypes.POINTER(NativePairResult), ctypes.c_int]
+                                        ctypes.POINTER(NativePairResult), ctypes.c_int,
+                                        ctypes.c_int]
     lib.indialign_search_ex.restype = ctypes.c_int
     lib.indialign_cuda_available.argtypes = []
     lib.indialign_cuda_available.restype = ctypes.c_int
     lib.indialign_batch_load.argtypes = [ctypes.c_int, ctypes.POINTER(NativePairInput)]
     lib.indialign_batch_load.restype = ctypes.c_int
     lib.indialign_batch_run.argtypes = [ctypes.POINTER(NativeConfig), ctypes.POINTER(NativePairResult),
-                                        ctypes.c_int]
+                                        ctypes.c_int, ctypes.c_int]
     lib.indialign_batch_run.restype = ctypes.c_int
     lib.indialign_last_stats.argtypes = [c_double_p, ctypes.c_int]
     lib.indialign_last_stats.restype = ctypes.c_int
@@ -155,19 +156,25 @@ def tmscore_search(pred, native, valid=None, pred_valid=None, native_valid=None,
     return float(s[0]), R[0], t[0]
 
 
-def tmscore_search_batch(pairs, cfg: NativeConfig | None = None, serial_bank: int | None = None):
+METHOD_PLAIN, METHOD_FAST = 0, 1
+
+
+def tmscore_search_batch(pairs, cfg: NativeConfig | None = None, serial_bank: int | None = None,
+                         method: int = METHOD_PLAIN):
     """Many pairs in one batched call.  `pairs` holds synth.Pair objects or
     (pred, native, valid, pred_valid, native_valid, lnorm) tuples.  Returns
     (scores[n], R[n,3,3], t[n,3]).  serial_bank None uses the batch entry's
-    convention (early termination applies); 0/1 selects it explicitly."""
+    convention (early termination applies); 0/1 selects it explicitly.
+    method: METHOD_PLAIN (FP64, reference arithmetic) or METHOD_FAST."""
     lib = load_library()
     cfg = cfg or default_config()
     pk = _Packed([_as_tuple(p) for p in pairs])
     res = (NativePairResult * max(1, pk.n))()
-    if serial_bank is None:
+    if serial_bank is None and method == METHOD_PLAIN:
         rc = lib.indialign_tmscore_search_batch(pk.n, pk.inps, ctypes.byref(cfg), res)
     else:
-        rc = lib.indialign_search_ex(pk.n, pk.inps, ctypes.byref(cfg), res, int(serial_bank))
+        sb = 1 if serial_bank is None else int(serial_bank)
+        rc = lib.indialign_search_ex(pk.n, pk.inps, ctypes.byref(cfg), res, sb, int(method))
     if rc != 0:
         raise IndialignError(f"batch search returned {rc}")
     return _results(res, pk.n)
@@ -187,11 +194,11 @@ def batch_load(pairs) -> int:
     return pk.n
 
 
-def batch_run(cfg: NativeConfig | None = None, serial_bank: int = 1):
+def batch_run(cfg: NativeConfig | None = None, serial_bank: int = 1, method: int = METHOD_PLAIN):
     """Search the resident batch (indialign_batch_run); (scores, R, t)."""
     cfg = cfg or default_config()
     res = (NativePairResult * max(1, _resident_n))()
-    rc = load_library().indialign_batch_run(ctypes.byref(cfg), res, int(serial_bank))
+    rc = load_library().indialign_batch_run(ctypes.byref(cfg), res, int(serial_bank), int(method))
     if rc != 0:
         raise IndialignError(f"indialign_batch_run returned {rc}")
     return _results(res, _resident_n)

@@ -148,7 +148,9 @@ class Engine {
     // Copy a batch into device memory (kept resident until the next load).
     void load(const NativePairInput *inp, int n);
     // Search the resident batch; out has one slot per loaded pair.
-    int search(const NativeConfig *cfg, bool serial_bank, NativePairResult *out);
+    // method 0: plain FP64 (bit-identical to the reference's arithmetic);
+    // method 1: fast (FP32 NW fill, closed-form FP64 rotations).
+    int search(const NativeConfig *cfg, bool serial_bank, NativePairResult *out, int method = 0);
     int loaded() const { return n_loaded_; }
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
@@ -158,6 +160,7 @@ class Engine {
     int n_cu_ = 256;
     const NativeConfig *cfg_ = nullptr;
     bool serial_ = true;
+    bool fast_ = false;
     std::vector<HostPair> hp_;
     std::vector<Hit> best_;
     std::vector<int> live_;  // pairs still in the pipeline (index into hp_)
@@ -198,7 +201,7 @@ class Engine {
 
     // arena
     DBuf b_px, b_py, b_pz, b_qx, b_qy, b_qz, b_v, b_pv, b_nv, b_pcx, b_pcy, b_pcz, b_qcx, b_qcy,
-        b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info;
+        b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info, b_qfx, b_qfy, b_qfz;
     // work buffers
     DBuf b_seeds, b_masks, b_banks, b_blk_bank, b_blk_first, b_cands, b_bank_out, b_lists_p,
         b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
@@ -208,6 +211,7 @@ class Engine {
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
                 &b_pcz, &b_qcx, &b_qcy, &b_qcz, &b_pmap, &b_nmap, &b_ssp, &b_ssn, &b_info,
+                &b_qfx, &b_qfy, &b_qfz,
                 &b_seeds, &b_masks, &b_banks, &b_blk_bank, &b_blk_first, &b_cands,
                 &b_bank_out, &b_lists_p, &b_lists_n, &b_list_na, &b_lout, &b_groups, &b_xf,
                 &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
@@ -348,6 +352,14 @@ void Engine::prepare(const NativePairInput *inp, int n) {
     A_.pcx = upload(b_pcx, pcx); A_.pcy = upload(b_pcy, pcy); A_.pcz = upload(b_pcz, pcz);
     A_.qcx = upload(b_qcx, qcx); A_.qcy = upload(b_qcy, qcy); A_.qcz = upload(b_qcz, qcz);
     A_.pmap = upload(b_pmap, pmap); A_.nmap = upload(b_nmap, nmap);
+    {   // FP32 copies of the compact native coordinates (fast method)
+        std::vector<float> fx(qcx.begin(), qcx.end()), fy(qcy.begin(), qcy.end()),
+            fz(qcz.begin(), qcz.end());
+        A_.qfx = upload(b_qfx, fx);
+        A_.qfy = upload(b_qfy, fy);
+        A_.qfz = upload(b_qfz, fz);
+        IA_CHECK(hipStreamSynchronize(s_));
+    }
     A_.ssp = b_ssp.get<char>(std::max(off, 1));
     A_.ssn = b_ssn.get<char>(std::max(off, 1));
     A_.info = upload(b_info, info);
@@ -408,7 +420,7 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
     IA_CHECK(hipEventRecord(e0_, s_));
     launch_bank(s_, A_, db, dbb, dbf, (int)blk_bank.size(), ds, b_masks.get<uint8_t>(1),
                 b_lists_p.get<int>(1), b_lists_n.get<int>(1), b_list_na.get<int>(1), list_cap_,
-                dc);
+                dc, fast_);
     launched();
     IA_CHECK(hipGetLastError());
     IA_CHECK(hipEventRecord(e1_, s_));
@@ -452,7 +464,8 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     // stage the columns only when that costs no resident wavefront
     const size_t lds_c = chain_lds_bytes(mp, mn, in_lds, true);
     const size_t lds_g = chain_lds_bytes(mp, mn, in_lds, false);
-    const bool cols_lds = lds_c <= kLdsMax && kLdsMax / lds_c >= std::min<size_t>(8, kLdsMax / lds_g);
+    const bool cols_lds = !fast_ && lds_c <= kLdsMax &&
+                          kLdsMax / lds_c >= std::min<size_t>(8, kLdsMax / lds_g);
     const size_t lds = cols_lds ? lds_c : lds_g;
     if (lds > kLdsMax) throw DevError("chain too long for one workgroup's LDS");
     int per_cu = std::max(1, chain_blocks_per_cu(lds));
@@ -471,7 +484,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     launch_chains(s_, A_, dg, (int)groups.size(), total, xf_dev, out_dev,
                   b_lout.get<ListOut>(1), b_lists_p.get<int>(1), b_lists_n.get<int>(1),
                   b_list_na.get<int>(1), list_cap_, mp, mn, q, trace, slot_words, (int)slots,
-                  cols_lds);
+                  cols_lds, fast_);
     launched();
     IA_CHECK(hipGetLastError());
     IA_CHECK(hipEventRecord(e1_, s_));
@@ -1109,11 +1122,13 @@ void Engine::load(const NativePairInput *inp, int n) {
     prepare(inp, n);
 }
 
-int Engine::search(const NativeConfig *cfg, bool serial_bank, NativePairResult *out) {
+int Engine::search(const NativeConfig *cfg, bool serial_bank, NativePairResult *out,
+                   int method) {
     const auto t0 = std::chrono::steady_clock::now();
     for (double &x : stats) x = 0;
     cfg_ = cfg;
     serial_ = serial_bank;
+    fast_ = method == 1;
     live_.clear();
     best_.assign(hp_.size(), identity_hit(kNeg));
     // Lnorm <= 2 -> score 0, identity (indialign.cpp:47-51)
@@ -1151,7 +1166,7 @@ std::unique_ptr<ia::Engine> g_engine;
 double g_stats[8];
 
 int run_search(int n, const NativePairInput *inp, const NativeConfig *cfg, NativePairResult *out,
-               int serial) {
+               int serial, int method = 0) {
     const char *env = std::getenv("INDIALIGN_SERIAL_BANK");
     if (env && *env) serial = std::atoi(env) != 0;
     std::lock_guard<std::mutex> lk(g_mu);
@@ -1160,7 +1175,8 @@ int run_search(int n, const NativePairInput *inp, const NativeConfig *cfg, Nativ
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
             throw ia::DevError("no HIP device available");
         if (!g_engine) g_engine.reset(new ia::Engine());
-        const int rc = g_engine->run(inp, n, cfg, serial != 0, out);
+        g_engine->load(inp, n);
+        const int rc = g_engine->search(cfg, serial != 0, out, method);
         std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
         return rc;
     } catch (const std::exception &e) {
@@ -1198,9 +1214,10 @@ int indialign_tmscore_search_batch(int n_pairs, const NativePairInput *inputs,
 }
 
 int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
-                        NativePairResult *outputs, int serial_bank) {
+                        NativePairResult *outputs, int serial_bank, int method) {
     if (n_pairs <= 0) return 0;
-    return run_search(n_pairs, inputs, cfg, outputs, serial_bank);
+    if (method != 0 && method != 1) return -3;
+    return run_search(n_pairs, inputs, cfg, outputs, serial_bank, method);
 }
 
 int indialign_batch_load(int n_pairs, const NativePairInput *inputs) {
@@ -1219,11 +1236,13 @@ int indialign_batch_load(int n_pairs, const NativePairInput *inputs) {
     }
 }
 
-int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int serial_bank) {
+int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int serial_bank,
+                        int method) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_engine) return -2;
+    if (method != 0 && method != 1) return -3;
     try {
-        const int rc = g_engine->search(cfg, serial_bank != 0, outputs);
+        const int rc = g_engine->search(cfg, serial_bank != 0, outputs, method);
         std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
         return rc;
     } catch (const std::exception &e) {

@@ -42,6 +42,7 @@ struct Arena {
     const uint8_t *v, *pv, *nv;
     const double *pcx, *pcy, *pcz, *qcx, *qcy, *qcz;  // compact by pv / nv
     const int *pmap, *nmap;                            // compact -> full index
+    const float *qfx, *qfy, *qfz;                      // compact native, FP32 (fast method)
     const char *ssp, *ssn;                             // secondary structure (full index)
     const PairInfo *info;
 };
@@ -160,6 +161,95 @@ IA_HD void horn_rotation(const double h[9], double *R) {
     R[8] = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
 }
 
+// 3x3 minor of a 4x4 matrix without row I and column J.
+template <int I, int J>
+IA_HD double minor3(const double m[4][4]) {
+    constexpr int r0 = I == 0 ? 1 : 0, r1 = I <= 1 ? 2 : 1, r2 = I <= 2 ? 3 : 2;
+    constexpr int c0 = J == 0 ? 1 : 0, c1 = J <= 1 ? 2 : 1, c2 = J <= 2 ? 3 : 2;
+    return m[r0][c0] * (m[r1][c1] * m[r2][c2] - m[r1][c2] * m[r2][c1]) -
+           m[r0][c1] * (m[r1][c0] * m[r2][c2] - m[r1][c2] * m[r2][c0]) +
+           m[r0][c2] * (m[r1][c0] * m[r2][c1] - m[r1][c1] * m[r2][c0]);
+}
+
+// Row I of adj(M) for symmetric M; keeps it in q if its norm is the largest.
+template <int I>
+IA_HD void adj_row_best(const double m[4][4], double &bn, double *q) {
+    constexpr double s0 = (I % 2 == 0) ? 1.0 : -1.0;
+    const double a = s0 * minor3<I, 0>(m), b = -s0 * minor3<I, 1>(m);
+    const double c = s0 * minor3<I, 2>(m), d = -s0 * minor3<I, 3>(m);
+    const double nn = a * a + b * b + c * c + d * d;
+    if (nn > bn) {
+        bn = nn;
+        q[0] = a; q[1] = b; q[2] = c; q[3] = d;
+    }
+}
+
+// Closed-form alternative to horn_rotation for the fast method (FP64):
+// the largest eigenvalue of the traceless Horn key matrix F is the largest
+// root of its characteristic quartic x^4 + c2 x^2 + c1 x + c0 (Theobald,
+// Acta Cryst. A61:478, 2005), found by Newton's method from the upper bound
+// sqrt(3)*|H|_F >= s1 + s2 + s3; the eigenvector is the largest row of
+// adj(F - x I).  Returns false when the spectrum is too degenerate for the
+// adjugate (the caller falls back to horn_rotation).
+IA_HD bool qcp_rotation(const double h[9], double *R) {
+    double m[4][4];
+    m[0][0] = h[0] + h[4] + h[8];
+    m[0][1] = m[1][0] = h[5] - h[7];
+    m[0][2] = m[2][0] = h[6] - h[2];
+    m[0][3] = m[3][0] = h[1] - h[3];
+    m[1][1] = h[0] - h[4] - h[8];
+    m[1][2] = m[2][1] = h[1] + h[3];
+    m[1][3] = m[3][1] = h[6] + h[2];
+    m[2][2] = -h[0] + h[4] - h[8];
+    m[2][3] = m[3][2] = h[5] + h[7];
+    m[3][3] = -h[0] - h[4] + h[8];
+    double n2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) n2 += h[k] * h[k];
+    if (!(n2 > 1e-300)) return false;
+    const double det3 = h[0] * (h[4] * h[8] - h[5] * h[7]) - h[1] * (h[3] * h[8] - h[5] * h[6]) +
+                        h[2] * (h[3] * h[7] - h[4] * h[6]);
+    const double detF = m[0][0] * minor3<0, 0>(m) - m[0][1] * minor3<0, 1>(m) +
+                        m[0][2] * minor3<0, 2>(m) - m[0][3] * minor3<0, 3>(m);
+    const double c2 = -2.0 * n2, c1 = -8.0 * det3, c0 = detF;
+    double x = sqrt(3.0 * n2);
+    for (int it = 0; it < 60; it++) {
+        const double x2 = x * x, b = (x2 + c2) * x, a = b + c1;
+        const double d = (a * x + c0) / (2.0 * x2 * x + b + a);
+        x -= d;
+        if (fabs(d) <= 1e-14 * fabs(x)) break;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) m[i][i] -= x;
+    double bn = 0.0, q[4] = {1.0, 0.0, 0.0, 0.0};
+    adj_row_best<0>(m, bn, q);
+    adj_row_best<1>(m, bn, q);
+    adj_row_best<2>(m, bn, q);
+    adj_row_best<3>(m, bn, q);
+    // a tiny adjugate means a (near) repeated top eigenvalue
+    if (!(bn > 1e-20 * n2 * n2 * n2)) return false;
+    const double inv = 1.0 / sqrt(bn);
+    const double q0 = q[0] * inv, q1 = q[1] * inv, q2 = q[2] * inv, q3 = q[3] * inv;
+    R[0] = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
+    R[1] = 2 * (q1 * q2 + q0 * q3);
+    R[2] = 2 * (q1 * q3 - q0 * q2);
+    R[3] = 2 * (q1 * q2 - q0 * q3);
+    R[4] = q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3;
+    R[5] = 2 * (q2 * q3 + q0 * q1);
+    R[6] = 2 * (q1 * q3 + q0 * q2);
+    R[7] = 2 * (q2 * q3 - q0 * q1);
+    R[8] = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+    return true;
+}
+
+// Rotation from a correlation: exact Jacobi path, or the closed form with a
+// Jacobi fallback.
+template <bool FAST>
+IA_HD void solve_rotation(const double h[9], double *R) {
+    if (FAST && qcp_rotation(h, R)) return;
+    horn_rotation(h, R);
+}
+
 // t = cQ - cP.R (core.h:121-122).
 IA_HD void finish_t(const double *cp, const double *cq, const double *R, double *t) {
     t[0] = cq[0] - (cp[0] * R[0] + cp[1] * R[3] + cp[2] * R[6]);
@@ -194,8 +284,9 @@ struct KabschSeq {
         h[3] += d1 * e0; h[4] += d1 * e1; h[5] += d1 * e2;
         h[6] += d2 * e0; h[7] += d2 * e1; h[8] += d2 * e2;
     }
+    template <bool FAST = false>
     IA_HD void solve(double *R, double *t) const {
-        horn_rotation(h, R);
+        solve_rotation<FAST>(h, R);
         finish_t(cp, cq, R, t);
     }
 };

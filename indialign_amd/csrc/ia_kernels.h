@@ -81,7 +81,7 @@ struct ListOut {
 void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
                  const int *blk_first, int n_blocks, const SeedDesc *seeds,
                  const uint8_t *masks, const int *lists_p, const int *lists_n,
-                 const int *list_na, int list_cap, Hit *cands);
+                 const int *list_na, int list_cap, Hit *cands, bool fast);
 void launch_bank_reduce(hipStream_t s, const BankDesc *banks, int n_banks, const int *list_na,
                         const Hit *cands, Hit *out);
 void launch_group_reduce(hipStream_t s, const long long *offs, const int *counts, int n,
@@ -101,7 +101,7 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
-                   int n_slots, bool cols_in_lds);
+                   int n_slots, bool cols_in_lds, bool fast);
 size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds);
 int chain_blocks_per_cu(size_t lds);
 

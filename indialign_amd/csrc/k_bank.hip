@@ -47,7 +47,7 @@ IA_D int bank_seeds(const BankDesc &B, int n) {
 // One candidate: iterative_seed_refine (search.h:28-56) or the gapless
 // offset start + refine_from_rt (threading.h:15-37, 415-430), then the
 // TM-score of search.h:358-362.
-template <int PTS>
+template <int PTS, bool FAST>
 IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const SeedDesc *seeds,
                         const uint8_t *masks, const int *lp, const int *ln, Hit &out) {
     const PairInfo &pi = A.info[B.pair];
@@ -87,7 +87,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
             ks.add_cov(A.px[off + i], A.py[off + i], A.pz[off + i], A.qx[off + jj],
                        A.qy[off + jj], A.qz[off + jj]);
         }
-        ks.solve(R, t);
+        ks.template solve<FAST>(R, t);
     } else {
         // seed & valid (search.h:34-35)
         const bool win = sd.b > 0;
@@ -110,7 +110,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
                 load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
                 ks.add_cov(x, y, z, qa, qb, qc);
             }
-            ks.solve(R, t);
+            ks.template solve<FAST>(R, t);
         }
     }
     // d0s-cutoff refinement; stop when the inlier count repeats.
@@ -137,7 +137,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
                 load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
                 if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) ks.add_cov(x, y, z, qa, qb, qc);
             }
-            ks.solve(R, t);
+            ks.template solve<FAST>(R, t);
         }
         if (cnt == last) break;
         last = cnt;
@@ -159,6 +159,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
     out.score = acc / fmax(pi.Lnorm, 1.0);
 }
 
+template <bool FAST>
 __global__ __launch_bounds__(256) void bank_kernel(Arena A, const BankDesc *banks,
                                                    const int *blk_bank, const int *blk_first,
                                                    const SeedDesc *seeds, const uint8_t *masks,
@@ -175,12 +176,12 @@ __global__ __launch_bounds__(256) void bank_kernel(Arena A, const BankDesc *bank
         h.score = -INFINITY;
         set_identity(h.R, h.t);
     } else if (B.pts == PTS_FULL) {
-        run_candidate<PTS_FULL>(A, B, n, j, seeds, masks, nullptr, nullptr, h);
+        run_candidate<PTS_FULL, FAST>(A, B, n, j, seeds, masks, nullptr, nullptr, h);
     } else if (B.pts == PTS_LIST) {
         const long long base = (long long)B.a0 * list_cap;
-        run_candidate<PTS_LIST>(A, B, n, j, seeds, masks, lists_p + base, lists_n + base, h);
+        run_candidate<PTS_LIST, FAST>(A, B, n, j, seeds, masks, lists_p + base, lists_n + base, h);
     } else {
-        run_candidate<PTS_CWIN>(A, B, n, j, seeds, masks, nullptr, nullptr, h);
+        run_candidate<PTS_CWIN, FAST>(A, B, n, j, seeds, masks, nullptr, nullptr, h);
     }
     cands[B.cand_off + j] = h;
 }
@@ -382,10 +383,14 @@ __global__ __launch_bounds__(64) void weighted_kernel(Arena A, const int *pairs,
 void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
                  const int *blk_first, int n_blocks, const SeedDesc *seeds,
                  const uint8_t *masks, const int *lists_p, const int *lists_n,
-                 const int *list_na, int list_cap, Hit *cands) {
+                 const int *list_na, int list_cap, Hit *cands, bool fast) {
     if (n_blocks <= 0) return;
-    bank_kernel<<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks, lists_p,
-                                         lists_n, list_na, list_cap, cands);
+    if (fast)
+        bank_kernel<true><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks,
+                                                   lists_p, lists_n, list_na, list_cap, cands);
+    else
+        bank_kernel<false><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks,
+                                                    lists_p, lists_n, list_na, list_cap, cands);
 }
 
 void launch_bank_reduce(hipStream_t s, const BankDesc *banks, int n_banks, const int *list_na,

@@ -18,6 +18,7 @@
 #include "ia_kernels.h"
 
 #include <math.h>
+#include <type_traits>
 
 namespace ia {
 
@@ -57,7 +58,7 @@ IA_D int group8_isum(int v) {
 
 // Wave-parallel two-pass Kabsch on n point pairs produced by get(q, ...).
 // Every lane ends with the same R, t (xor butterflies are order-symmetric).
-template <class Get>
+template <bool FAST, class Get>
 IA_D void wave_kabsch(int n, Get get, double *R, double *t) {
     const int lane = threadIdx.x & 63;
     double c[6] = {0, 0, 0, 0, 0, 0};
@@ -83,7 +84,7 @@ IA_D void wave_kabsch(int n, Get get, double *R, double *t) {
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) h[k] = wave_sum(h[k]);
-    horn_rotation(h, R);
+    solve_rotation<FAST>(h, R);
     finish_t(cp, cq, R, t);
 }
 
@@ -125,21 +126,27 @@ IA_D double ror1(double v) {
 // LDS.  Each cell's value H is max(diag, up, left) (ties only decide the
 // trace code), and G = H + gap_open when the cell was entered diagonally,
 // else H, is what its down and right neighbours add.
-template <int SMODE, class TR, bool CL, int K>
+template <int SMODE, class TR, bool CL, int K, class V>
 IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const double *t,
-                  double gap, double cut8, double d0sq_s, ChainLds &L, TR *trace) {
+                  double gap_d, double cut8_d, double d0sq_s, ChainLds &L, TR *trace) {
     const int lane = threadIdx.x & 63;
     const int plen = pi.plen, nlen = pi.nlen, off = pi.off;
     const int wpr = (nlen + 15) >> 4;
-    auto colx = [&](int b) { return CL ? L.colx[b] : A.qcx[off + b]; };
-    auto coly = [&](int b) { return CL ? L.coly[b] : A.qcy[off + b]; };
-    auto colz = [&](int b) { return CL ? L.colz[b] : A.qcz[off + b]; };
-    auto cols = [&](int b) { return CL ? L.cols[b] : A.ssn[off + A.nmap[off + b]]; };
+    constexpr bool F32 = sizeof(V) == 4;
+    // column operands: FP32 copies (fast method), the wavefront's LDS copy,
+    // or global memory for chains too long to stage
+    auto colx = [&](int b) -> V { if constexpr (F32) return A.qfx[off + b]; else return CL ? L.colx[b] : A.qcx[off + b]; };
+    auto coly = [&](int b) -> V { if constexpr (F32) return A.qfy[off + b]; else return CL ? L.coly[b] : A.qcy[off + b]; };
+    auto colz = [&](int b) -> V { if constexpr (F32) return A.qfz[off + b]; else return CL ? L.colz[b] : A.qcz[off + b]; };
+    auto cols = [&](int b) { return (CL && !F32) ? L.cols[b] : A.ssn[off + A.nmap[off + b]]; };
+    const V gap = (V)gap_d, cut8 = (V)cut8_d;
+    const V inv_d0sq = (V)(1.0 / d0sq_s);
+    V *rowHv = (V *)L.rowH;
     for (int s0 = 0; s0 < plen; s0 += 64 * K) {
         const int rows = min(64 * K, plen - s0);
         const bool first = s0 == 0;
         const bool has_next = s0 + 64 * K < plen;
-        double mx[K], my[K], mz[K], H[K], G[K], D[K], cx[K], cy[K], cz[K];
+        V mx[K], my[K], mz[K], H[K], G[K], D[K], cx[K], cy[K], cz[K];
         char sp[K], cs[K];
         bool rok[K];
         unsigned tw[K];
@@ -150,7 +157,11 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
             mx[j] = my[j] = mz[j] = 0.0;
             sp[j] = 0;
             if (rok[j]) {
-                if (SMODE != SM_SS) xform(R, t, A.pcx[off + a], A.pcy[off + a], A.pcz[off + a], mx[j], my[j], mz[j]);
+                if (SMODE != SM_SS) {
+                    double ex, ey, ez;
+                    xform(R, t, A.pcx[off + a], A.pcy[off + a], A.pcz[off + a], ex, ey, ez);
+                    mx[j] = (V)ex; my[j] = (V)ey; mz[j] = (V)ez;
+                }
                 if (SMODE != SM_TM) sp[j] = A.ssp[off + A.pmap[off + a]];
             }
             H[j] = G[j] = D[j] = 0.0;
@@ -162,13 +173,13 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
             if (SMODE != SM_TM) cs[j] = cols(b);
         }
         const int steps = nlen + rows - 1;
-        double rH = first ? 0.0 : L.rowH[0];
+        V rH = first ? (V)0 : rowHv[0];
         int rT = first ? -1 : (int)L.rowT[0];
         for (int st = 0; st < steps; st++) {
             const int rn = min(st + 1, nlen - 1);
-            const double rHn = first ? 0.0 : L.rowH[rn];
+            const V rHn = first ? (V)0 : rowHv[rn];
             const int rTn = first ? -1 : (int)L.rowT[rn];
-            double XH[K], XG[K];
+            V XH[K], XG[K];
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 XH[j] = ror1(H[j]);
@@ -180,29 +191,32 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
                 const bool act = rok[j] && b >= 0 && b < nlen;
                 // next step's column operands
                 const int bn = min(max(b + 1, 0), nlen - 1);
-                double nx = 0, ny = 0, nz = 0;
+                V nx = 0, ny = 0, nz = 0;
                 char ns = 0;
                 if (SMODE != SM_SS) { nx = colx(bn); ny = coly(bn); nz = colz(bn); }
                 if (SMODE != SM_TM) ns = cols(bn);
-                double s;
+                V s;
                 if (SMODE == SM_SS) {
-                    s = (sp[j] == cs[j]) ? 1.0 : 0.0;
+                    s = (sp[j] == cs[j]) ? (V)1 : (V)0;
                 } else {
-                    const double dx = mx[j] - cx[j], dy = my[j] - cy[j], dz = mz[j] - cz[j];
-                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    const V dx = mx[j] - cx[j], dy = my[j] - cy[j], dz = mz[j] - cz[j];
+                    const V d2 = dx * dx + dy * dy + dz * dz;
+                    V tm;
+                    if constexpr (F32) tm = __builtin_amdgcn_rcpf(1.0f + d2 * inv_d0sq);
+                    else tm = 1.0 / (1.0 + d2 / d0sq_s);
                     if (SMODE == SM_TM) {
-                        s = (d2 <= cut8) ? 1.0 / (1.0 + d2 / d0sq_s) : 0.0;
+                        s = (d2 <= cut8) ? tm : (V)0;
                     } else {
-                        s = 1.0 / (1.0 + d2 / d0sq_s);
-                        if (sp[j] == cs[j]) s += 0.5;
+                        s = tm;
+                        if (sp[j] == cs[j]) s += (V)0.5;
                     }
                 }
                 // up operands (raw H for the next diagonal, G for "up")
-                double hu, gu;
+                V hu, gu;
                 if (j == 0) {
                     const bool bound = first || b >= nlen;
-                    const double h0 = bound ? 0.0 : rH;
-                    const double g0 = bound ? 0.0 : ((rT == 0) ? rH + gap : rH);
+                    const V h0 = bound ? (V)0 : rH;
+                    const V g0 = bound ? (V)0 : ((rT == 0) ? rH + gap : rH);
                     hu = lane == 0 ? h0 : XH[0];
                     gu = lane == 0 ? g0 : XG[0];
                 } else {
@@ -210,11 +224,11 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
                     gu = lane == 0 ? XG[j > 0 ? j - 1 : 0] : XG[j];
                 }
                 const bool edge = b == 0;
-                const double diag = (edge ? 0.0 : D[j]) + s;
-                const double left = edge ? 0.0 : G[j];
-                const double hm = fmax(diag, fmax(gu, left));
+                const V diag = (edge ? (V)0 : D[j]) + s;
+                const V left = edge ? (V)0 : G[j];
+                const V hm = fmax(diag, fmax(gu, left));
                 const int tn = (diag >= gu && diag >= left) ? 0 : (left >= gu ? 2 : 1);
-                const double gn = (tn == 0) ? hm + gap : hm;
+                const V gn = (tn == 0) ? hm + gap : hm;
                 if (act) {
                     tw[j] |= (unsigned)tn << ((b & 15) * 2);
                     if ((b & 15) == 15 || b == nlen - 1) {
@@ -222,7 +236,7 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
                         tw[j] = 0;
                     }
                     if (j == K - 1 && lane == 63 && has_next) {
-                        L.rowH[b] = hm;
+                        rowHv[b] = hm;
                         L.rowT[b] = (signed char)tn;
                     }
                 }
@@ -241,16 +255,17 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
 
 // Needleman-Wunsch fill + traceback; returns the aligned count (pairs in
 // alP/alN, reverse order).  nw.h:10-68 semantics.
-template <int SMODE, class TR, bool CL>
+template <int SMODE, class TR, bool CL, bool FAST>
 IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const double *t,
                   double gap, double cut8, double d0sq_s, ChainLds &L, TR *trace) {
     const int lane = threadIdx.x & 63;
     const int plen = pi.plen, nlen = pi.nlen;
     const int wpr = (nlen + 15) >> 4;
+    typedef typename std::conditional<FAST, float, double>::type V;
     if (plen <= 64)
-        nw_fill<SMODE, TR, CL, 1>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
+        nw_fill<SMODE, TR, CL, 1, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
     else
-        nw_fill<SMODE, TR, CL, 2>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
+        nw_fill<SMODE, TR, CL, 2, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
     __builtin_amdgcn_wave_barrier();
     // traceback (nw.h:42-68) by lane 0
@@ -281,6 +296,7 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
 // Lite detailed search (align.h:15-50 with use_frag = false): the whole
 // aligned set as the only seed, cutoffs d0s x {2,1.5,1,.75,.5,.25}, 8
 // refinement rounds, TM-score with the d8 cut.  Eight lanes per cutoff.
+template <bool FAST>
 IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int na,
                       const double *nxR, const double *nxt, double sd8, Hit &det) {
     const int lane = threadIdx.x & 63;
@@ -336,7 +352,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
 #pragma unroll
         for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
         if (solve) {
-            horn_rotation(h, R);
+            solve_rotation<FAST>(h, R);
             finish_t(cp, cq, R, t);
         }
         if (!done) {
@@ -382,7 +398,7 @@ IA_D int find_group(const ChainGroup *groups, int n_groups, long long task) {
     return lo;
 }
 
-template <int SMODE, class TR, bool CL>
+template <int SMODE, class TR, bool CL, bool FAST>
 IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int c,
                     const Hit *xf, Hit *out, ListOut *lout, int *lists_p, int *lists_n,
                     int *list_na, int list_cap, ChainLds &L, TR *trace) {
@@ -416,7 +432,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
         const int r = item - G.items_prefix[L_];
         const int flen = G.lens[L_];
         const int ps = (r / G.n_ns[L_]) * G.jp, ns = (r % G.n_ns[L_]) * G.jn;
-        wave_kabsch(flen,
+        wave_kabsch<FAST>(flen,
                     [&](int q, double &x, double &y, double &z, double &a, double &b, double &cc) {
                         x = A.pcx[off + ps + q]; y = A.pcy[off + ps + q]; z = A.pcz[off + ps + q];
                         a = A.qcx[off + ns + q]; b = A.qcy[off + ns + q]; cc = A.qcz[off + ns + q];
@@ -461,7 +477,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             }
 #pragma unroll
             for (int q = 0; q < 9; q++) h[q] = wave_sum(h[q]);
-            horn_rotation(h, R);
+            solve_rotation<FAST>(h, R);
             finish_t(cp, cq, R, t);
         }
     }
@@ -483,13 +499,13 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     const double cut8 = G.sd8 * G.sd8;
     const double d0sq = fmax(pi.d0 * pi.d0, 1e-12);
     for (int it = 0; it < G.iters; it++) {
-        const int na = nw_align<SMODE, TR, CL>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
+        const int na = nw_align<SMODE, TR, CL, FAST>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
         if (G.mode == CM_LIST) {
             const int slot = G.list_slot0 + c;
             double nR[9], nt[3];
             double sc = 0.0;
             if (na >= 3) {
-                wave_kabsch(na,
+                wave_kabsch<FAST>(na,
                             [&](int q, double &x, double &y, double &z, double &a, double &b, double &cc) {
                                 const int ap = L.alP[q], an = L.alN[q];
                                 x = A.pcx[off + ap]; y = A.pcy[off + ap]; z = A.pcz[off + ap];
@@ -527,7 +543,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
         }
         if (na < 3) continue;  // align.h:106
         double nR[9], nt[3];
-        wave_kabsch(na,
+        wave_kabsch<FAST>(na,
                     [&](int q, double &x, double &y, double &z, double &a, double &b, double &cc) {
                         const int ap = L.alP[q], an = L.alN[q];
                         x = A.pcx[off + ap]; y = A.pcy[off + ap]; z = A.pcz[off + ap];
@@ -542,7 +558,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
         }
         const double it_score = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
         Hit det;
-        lite_search(A, pi, L, na, nR, nt, G.sd8, det);
+        lite_search<FAST>(A, pi, L, na, nR, nt, G.sd8, det);
         Hit step;
         if (det.score > it_score) {
             step = det;
@@ -560,7 +576,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     if (lane == 0) *dst = best;
 }
 
-template <class TR, bool CL>
+template <class TR, bool CL, bool FAST>
 IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, long long n_chains,
                      const Hit *xf, Hit *out, ListOut *lout, int *lists_p, int *lists_n,
                      int *list_na, int list_cap, int *queue, ChainLds &L, TR *trace) {
@@ -575,11 +591,11 @@ IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, lon
         const int c = (int)(task - G.chain_off);
         const PairInfo &pi = A.info[G.pair];
         if (G.smat == SM_TM)
-            run_chain<SM_TM, TR, CL>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+            run_chain<SM_TM, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         else if (G.smat == SM_SS)
-            run_chain<SM_SS, TR, CL>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+            run_chain<SM_SS, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         else
-            run_chain<SM_IA4, TR, CL>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+            run_chain<SM_IA4, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
         __builtin_amdgcn_wave_barrier();
     }
@@ -607,6 +623,7 @@ struct LdsPlan {
     }
 };
 
+template <bool FAST>
 __global__ __launch_bounds__(64) void chain_kernel(Arena A, const ChainGroup *groups,
                                                    int n_groups, long long n_chains,
                                                    const Hit *xf, Hit *out, ListOut *lout,
@@ -630,18 +647,18 @@ __global__ __launch_bounds__(64) void chain_kernel(Arena A, const ChainGroup *gr
     if (trace_g) {
         unsigned *tg = trace_g + (long long)blockIdx.x * slot_words;
         if (cols_in_lds)
-            chain_loop<unsigned, true>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+            chain_loop<unsigned, true, FAST>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
                                        lists_n, list_na, list_cap, queue, L, tg);
         else
-            chain_loop<unsigned, false>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+            chain_loop<unsigned, false, FAST>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
                                         lists_n, list_na, list_cap, queue, L, tg);
     } else {
         typedef __attribute__((address_space(3))) unsigned lds_u32;
         if (cols_in_lds)
-            chain_loop<lds_u32, true>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+            chain_loop<lds_u32, true, FAST>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
                                       lists_n, list_na, list_cap, queue, L, (lds_u32 *)L.trace);
         else
-            chain_loop<lds_u32, false>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+            chain_loop<lds_u32, false, FAST>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
                                        lists_n, list_na, list_cap, queue, L, (lds_u32 *)L.trace);
     }
 }
@@ -650,7 +667,7 @@ __global__ __launch_bounds__(64) void chain_kernel(Arena A, const ChainGroup *gr
 
 int chain_blocks_per_cu(size_t lds) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel, 64, lds) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<false>, 64, lds) != hipSuccess)
         return 1;
     return nb;
 }
@@ -663,13 +680,18 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
-                   int n_slots, bool cols_in_lds) {
+                   int n_slots, bool cols_in_lds, bool fast) {
     if (n_chains <= 0) return;
     const size_t lds = chain_lds_bytes(max_plen, max_nlen, trace_scratch == nullptr, cols_in_lds);
     const int grid = n_slots;
-    hipLaunchKernelGGL(chain_kernel, dim3(grid), dim3(64), lds, s, A, groups, n_groups, n_chains,
-                       xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen, max_nlen,
-                       queue, trace_scratch, trace_slot_words, (int)cols_in_lds);
+    if (fast)
+        hipLaunchKernelGGL(chain_kernel<true>, dim3(grid), dim3(64), lds, s, A, groups, n_groups,
+                           n_chains, xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen,
+                           max_nlen, queue, trace_scratch, trace_slot_words, 0);
+    else
+        hipLaunchKernelGGL(chain_kernel<false>, dim3(grid), dim3(64), lds, s, A, groups, n_groups,
+                           n_chains, xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen,
+                           max_nlen, queue, trace_scratch, trace_slot_words, (int)cols_in_lds);
 }
 
 }  // namespace ia
