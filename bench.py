@@ -16,6 +16,11 @@ baseline is the reference library compiled from source (oracle/_ref), batch
 entry with OpenMP over pairs on the host cores, on a bounded sample of the
 same workload.
 
+The `learned` object times method 2 on the same batch (FP32 NW fill, and
+of the local-fragment rescue work items only the top-64 per pair by the
+packaged trained ranker are refined) and gives its TM loss against the plain
+FP64 scores of the same pairs.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs P]
 """
 from __future__ import annotations
@@ -57,6 +62,34 @@ def cpu_baseline(pairs, sample):
                       f"reference batch entry, {dt:.1f} s"}
 
 
+def learned_leg(ia, cfg, plain_scores, args, plain_s_per_step):
+    """Method 2 (FP32 NW fill + the packaged trained ranker, top-m
+    local-fragment work items per pair) on the same resident batch; TM loss
+    against the plain FP64 scores of the timed steps."""
+    ia.load_default_ranker(top_m=args.top_m)
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        ia.batch_run(cfg, serial_bank=1, method=ia.METHOD_LEARNED)
+        log(f"[bench] learned warmup {w}: {time.perf_counter() - t:.2f} s")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        t = time.perf_counter()
+        s, _, _ = ia.batch_run(cfg, serial_bank=1, method=ia.METHOD_LEARNED)
+        log(f"[bench] learned step {k}: {time.perf_counter() - t:.2f} s")
+    el = (time.perf_counter() - t0) / args.steps
+    st = ia.last_stats()
+    ia.set_ranker(None)
+    d = plain_scores - s
+    n = len(s)
+    return {"method": f"FP32 NW fill + trained ranker, top-{args.top_m} local-fragment work items",
+            "value": round(n / el, 3), "unit": "pairs/s", "ms_per_step": round(1000 * el, 2),
+            "speedup_vs_plain": round(plain_s_per_step / el, 3),
+            "tm_loss_vs_plain": {"mean": float(d.mean()), "max": float(d.max()),
+                                 "pairs_over_1e-3": int((d > 1e-3).sum()),
+                                 "pairs_over_1e-4": int((d > 1e-4).sum())},
+            "nw_cells": st["nw_cells"], "chain_ms": round(st["chain_ms"], 2)}
+
+
 def measured_traffic():
     """HBM bytes per chain-kernel launch from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (profiles/chain_kernel_short512.md)."""
@@ -75,6 +108,8 @@ def main():
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--top-m", type=int, default=64)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy kernel of this size (counter calibration)")
     args = ap.parse_args()
@@ -147,6 +182,8 @@ def main():
                      "traffic": measured_traffic()},
         "mean_score": round(float(scores.mean()), 6),
     }
+    if not args.no_learned:
+        out["learned"] = learned_leg(ia, cfg, scores, args, elapsed / args.steps)
     if not args.no_cpu:
         log(f"[bench] cpu baseline on {args.cpu_sample} pairs ...")
         out["cpu_baseline"] = cpu_baseline(pairs, args.cpu_sample)

@@ -88,8 +88,11 @@ int indialign_cuda_available(void);
  * method = 0: plain FP64, the reference's arithmetic (results identical to
  * the reference CPU path up to its own rounding); method = 1: fast (FP32
  * Needleman-Wunsch fill, closed-form FP64 rotations; every reported score is
- * still an FP64 TM-score of the returned transform).  Same return codes as
- * the batch entry, -3 for an unknown method. */
+ * still an FP64 TM-score of the returned transform); method = 2: learned
+ * (method 1, and of the local-fragment rescue work items (rescue.h:60-118)
+ * of each pair only the top-m by the ranker installed with
+ * indialign_set_ranker are refined).  Same return codes as the batch entry,
+ * -3 for an unknown method, -4 for method 2 without a ranker. */
 int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
                         NativePairResult *outputs, int serial_bank, int method);
 
@@ -107,8 +110,59 @@ int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int 
  *   [0] wall ms  [1] candidates refined  [2] NW cells filled
  *   [3] chains run  [4] kernel launches
  *   [5] chain-kernel ms  [6] bank-kernel ms  (device time, HIP events)
+ *   [7] local-fragment passes that fell back to every work item (method 2)
  * Returns the number of values written. */
 int indialign_last_stats(double *out, int n);
+
+/* Running best score of each pair of the last search after each of the ten
+ * pipeline stages (indialign.cpp:69-264): 0 main seed bank, 1 SS seeds,
+ * 2 gapless threading, 3 local-fragment rescue, 4 threading DP, 5 dense
+ * local-fragment rescue, 6 top-k seed, 7 DP refinement, 8 offset DP,
+ * 9 weighted refinement; NaN where the stage did not run.  out receives
+ * n_pairs x 10 doubles, row-major by pair.  Returns the number of pairs
+ * written. */
+int indialign_last_stage_scores(double *out, int n_pairs);
+
+/* ---- learned ranking of the local-fragment work items ---------------------
+ * The ranker is an MLP 96 -> 256 -> 256 -> 1 (ReLU) over a work-item
+ * descriptor; its parameters are one float array of INDIALIGN_MLP_NPARAM
+ * values: W1[256][96], b1[256], W2[256][256], b2[256], w3[256], b3.  Matrix
+ * products use BF16 operands with FP32 accumulation. */
+#define INDIALIGN_MLP_NFEAT 96
+#define INDIALIGN_MLP_NPARAM 90881
+
+/* Install (params != NULL) or remove (NULL) the ranker used by method 2, the
+ * number of work items refined per pair and pass (top_m >= 1), and the
+ * fallback margin: a pair whose largest predicted refined score reaches its
+ * running best + fallback_margin refines every work item of that pass, as
+ * the plain search does (fallback_margin < 0: never).  The parameters are
+ * copied.  Returns 0, or -1 for top_m < 1. */
+int indialign_set_ranker(const float *params, int top_m, double fallback_margin);
+
+/* Plain search (method 0, serial seed bank) that also records every
+ * local-fragment work item: its descriptor row (X, INDIALIGN_MLP_NFEAT floats
+ * per row), its refined score (y, best of its two chains) and the index of
+ * its pair (pair_idx), up to cap rows; *n_records receives the count.
+ * Returns 0, -1 on bad arguments, -2 on a device error. */
+int indialign_collect_records(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
+                              NativePairResult *outputs, float *X, float *y, int *pair_idx,
+                              long long cap, long long *n_records);
+
+/* Ranker forward pass on the GPU over n host rows: y[i] = f(X[i]). */
+int indialign_mlp_forward(const float *params, const float *X, long long n, float *y);
+
+/* Gradient of the mean squared error mean_i (f(X[i]) - t[i])^2 over n rows
+ * (1 <= n <= 262144) with respect to params (grad, INDIALIGN_MLP_NPARAM
+ * floats) and the loss, from the GPU forward and backward kernels. */
+int indialign_mlp_grad(const float *params, const float *X, const float *t, int n, float *grad,
+                       float *loss);
+
+/* Adam training on the GPU (beta 0.9 / 0.999, eps 1e-8, constant lr):
+ * `epochs` passes over the n rows in minibatches of `batch` rows, shuffled by
+ * a seeded permutation; params is updated in place, epoch_loss (may be NULL)
+ * receives the mean minibatch loss of each epoch. */
+int indialign_mlp_train(float *params, const float *X, const float *t, long long n, int epochs,
+                        int batch, float lr, unsigned seed, float *epoch_loss);
 
 #ifdef __cplusplus
 }

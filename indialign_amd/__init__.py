@@ -85,6 +85,25 @@ def load_library(path: str | None = None):
     lib.indialign_batch_run.restype = ctypes.c_int
     lib.indialign_last_stats.argtypes = [c_double_p, ctypes.c_int]
     lib.indialign_last_stats.restype = ctypes.c_int
+    c_float_p = ctypes.POINTER(ctypes.c_float)
+    lib.indialign_last_stage_scores.argtypes = [c_double_p, ctypes.c_int]
+    lib.indialign_last_stage_scores.restype = ctypes.c_int
+    lib.indialign_set_ranker.argtypes = [c_float_p, ctypes.c_int, ctypes.c_double]
+    lib.indialign_set_ranker.restype = ctypes.c_int
+    lib.indialign_collect_records.argtypes = [
+        ctypes.c_int, ctypes.POINTER(NativePairInput), ctypes.POINTER(NativeConfig),
+        ctypes.POINTER(NativePairResult), c_float_p, c_float_p, ctypes.POINTER(ctypes.c_int),
+        ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]
+    lib.indialign_collect_records.restype = ctypes.c_int
+    lib.indialign_mlp_forward.argtypes = [c_float_p, c_float_p, ctypes.c_longlong, c_float_p]
+    lib.indialign_mlp_forward.restype = ctypes.c_int
+    lib.indialign_mlp_grad.argtypes = [c_float_p, c_float_p, c_float_p, ctypes.c_int, c_float_p,
+                                       c_float_p]
+    lib.indialign_mlp_grad.restype = ctypes.c_int
+    lib.indialign_mlp_train.argtypes = [c_float_p, c_float_p, c_float_p, ctypes.c_longlong,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                        ctypes.c_uint, c_float_p]
+    lib.indialign_mlp_train.restype = ctypes.c_int
     if path is None:
         _lib = lib
     return lib
@@ -156,7 +175,7 @@ def tmscore_search(pred, native, valid=None, pred_valid=None, native_valid=None,
     return float(s[0]), R[0], t[0]
 
 
-METHOD_PLAIN, METHOD_FAST = 0, 1
+METHOD_PLAIN, METHOD_FAST, METHOD_LEARNED = 0, 1, 2
 
 
 def tmscore_search_batch(pairs, cfg: NativeConfig | None = None, serial_bank: int | None = None,
@@ -165,7 +184,8 @@ def tmscore_search_batch(pairs, cfg: NativeConfig | None = None, serial_bank: in
     (pred, native, valid, pred_valid, native_valid, lnorm) tuples.  Returns
     (scores[n], R[n,3,3], t[n,3]).  serial_bank None uses the batch entry's
     convention (early termination applies); 0/1 selects it explicitly.
-    method: METHOD_PLAIN (FP64, reference arithmetic) or METHOD_FAST."""
+    method: METHOD_PLAIN (FP64, reference arithmetic), METHOD_FAST or
+    METHOD_LEARNED (needs set_ranker)."""
     lib = load_library()
     cfg = cfg or default_config()
     pk = _Packed([_as_tuple(p) for p in pairs])
@@ -208,5 +228,130 @@ def last_stats() -> dict:
     """Work counters of the last call (see indialign_last_stats in the header)."""
     buf = (ctypes.c_double * 8)()
     load_library().indialign_last_stats(buf, 8)
-    keys = ["wall_ms", "candidates", "nw_cells", "chains", "launches", "chain_ms", "bank_ms"]
+    keys = ["wall_ms", "candidates", "nw_cells", "chains", "launches", "chain_ms", "bank_ms",
+            "fallbacks"]
     return {k: buf[i] for i, k in enumerate(keys)}
+
+
+STAGES = ["seed_bank", "ss", "gapless", "local_fragment", "threading_dp", "dense_local_fragment",
+          "topk_seed", "dp_refine", "offset_dp", "weighted_refine"]
+
+
+def last_stage_scores(n: int) -> np.ndarray:
+    """Running best score of each pair after each pipeline stage of the last
+    search, shape (n, 10), NaN where a stage did not run (order: STAGES)."""
+    out = np.full((max(n, 1), len(STAGES)), np.nan)
+    m = load_library().indialign_last_stage_scores(out.ctypes.data_as(c_double_p), n)
+    return out[:m]
+
+
+# ---- learned ranking of the local-fragment work items -------------------------
+NFEAT = 96
+NPARAM = 90881
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def set_ranker(params, top_m: int = 64, fallback_margin: float = -1.0) -> None:
+    """Install the ranker (NPARAM float32 values) used by METHOD_LEARNED, or
+    remove it with params=None.  fallback_margin >= 0: a pair whose largest
+    predicted refined score reaches its running best + margin refines every
+    work item of that pass."""
+    if params is None:
+        load_library().indialign_set_ranker(None, 64, -1.0)
+        return
+    p = _f32(params).reshape(-1)
+    if p.size != NPARAM:
+        raise ValueError(f"ranker needs {NPARAM} parameters, got {p.size}")
+    rc = load_library().indialign_set_ranker(p.ctypes.data_as(_f32p), int(top_m),
+                                             float(fallback_margin))
+    if rc != 0:
+        raise IndialignError(f"indialign_set_ranker returned {rc}")
+
+
+def collect_records(pairs, cap: int, cfg: NativeConfig | None = None):
+    """Plain search that also returns every local-fragment work item's
+    descriptor X (rows x 96), refined score y and pair index.  Returns
+    (scores, R, t, X, y, pair_idx)."""
+    lib = load_library()
+    cfg = cfg or default_config()
+    pk = _Packed([_as_tuple(p) for p in pairs])
+    res = (NativePairResult * max(1, pk.n))()
+    X = np.empty((max(cap, 1), NFEAT), np.float32)
+    y = np.empty(max(cap, 1), np.float32)
+    pi = np.empty(max(cap, 1), np.int32)
+    nrec = ctypes.c_longlong(0)
+    rc = lib.indialign_collect_records(pk.n, pk.inps, ctypes.byref(cfg), res,
+                                       X.ctypes.data_as(_f32p), y.ctypes.data_as(_f32p),
+                                       pi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       int(cap), ctypes.byref(nrec))
+    if rc != 0:
+        raise IndialignError(f"indialign_collect_records returned {rc}")
+    m = nrec.value
+    s, R, t = _results(res, pk.n)
+    return s, R, t, X[:m].copy(), y[:m].copy(), pi[:m].copy()
+
+
+def mlp_forward(params, X) -> np.ndarray:
+    """Ranker forward pass on the GPU."""
+    p, Xf = _f32(params).reshape(-1), _f32(X).reshape(-1, NFEAT)
+    y = np.empty(Xf.shape[0], np.float32)
+    rc = load_library().indialign_mlp_forward(p.ctypes.data_as(_f32p), Xf.ctypes.data_as(_f32p),
+                                              Xf.shape[0], y.ctypes.data_as(_f32p))
+    if rc != 0:
+        raise IndialignError(f"indialign_mlp_forward returned {rc}")
+    return y
+
+
+def mlp_grad(params, X, t):
+    """(grad[NPARAM], loss) of the MSE from the GPU backward kernels."""
+    p, Xf, tf = _f32(params).reshape(-1), _f32(X).reshape(-1, NFEAT), _f32(t).reshape(-1)
+    g = np.empty(NPARAM, np.float32)
+    loss = np.zeros(1, np.float32)
+    rc = load_library().indialign_mlp_grad(p.ctypes.data_as(_f32p), Xf.ctypes.data_as(_f32p),
+                                           tf.ctypes.data_as(_f32p), Xf.shape[0],
+                                           g.ctypes.data_as(_f32p), loss.ctypes.data_as(_f32p))
+    if rc != 0:
+        raise IndialignError(f"indialign_mlp_grad returned {rc}")
+    return g, float(loss[0])
+
+
+def mlp_train(params, X, t, epochs: int, batch: int = 4096, lr: float = 1e-3, seed: int = 0):
+    """Adam training on the GPU; returns (new params, per-epoch loss)."""
+    p = _f32(params).reshape(-1).copy()
+    Xf, tf = _f32(X).reshape(-1, NFEAT), _f32(t).reshape(-1)
+    el = np.zeros(max(epochs, 1), np.float32)
+    rc = load_library().indialign_mlp_train(p.ctypes.data_as(_f32p), Xf.ctypes.data_as(_f32p),
+                                            tf.ctypes.data_as(_f32p), Xf.shape[0], int(epochs),
+                                            int(batch), float(lr), int(seed),
+                                            el.ctypes.data_as(_f32p))
+    if rc != 0:
+        raise IndialignError(f"indialign_mlp_train returned {rc}")
+    return p, el[:epochs]
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-uniform initial ranker parameters (biases zero)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NPARAM, np.float32)
+    w1, w2 = 256 * 96, 256 * 256
+    p[:w1] = rng.uniform(-1, 1, w1) * np.sqrt(6.0 / 96)
+    o = w1 + 256
+    p[o:o + w2] = rng.uniform(-1, 1, w2) * np.sqrt(6.0 / 256)
+    o += w2 + 256
+    p[o:o + 256] = rng.uniform(-1, 1, 256) * np.sqrt(6.0 / 256)
+    return p
+
+
+DEFAULT_RANKER = os.path.join(HERE, "ranker_params.npy")
+
+
+def load_default_ranker(top_m: int = 64, fallback_margin: float = -1.0,
+                        path: str | None = None) -> np.ndarray:
+    """Install the packaged trained ranker (train_ranker.py) for METHOD_LEARNED."""
+    p = np.load(path or DEFAULT_RANKER, allow_pickle=False)
+    set_ranker(p, top_m, fallback_margin)
+    return p

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libindialign.so")
 OBJ_DIR = os.path.join(HERE, "_obj")
-SOURCES = ["k_bank.hip", "k_chain.hip", "k_seed.hip", "engine.hip"]
+SOURCES = ["k_bank.hip", "k_chain.hip", "k_seed.hip", "k_learn.hip", "mlp.hip", "engine.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

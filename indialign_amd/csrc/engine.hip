@@ -6,8 +6,10 @@
 // take part in it; the host only enumerates work (seed windows, rescue work
 // items, offsets), applies the stage gates and performs the strict-'>' merges
 // on per-pair results copied back after each stage.
+#include "host_util.h"
 #include "ia_kernels.h"
 #include "indialign.h"
+#include "mlp.h"
 
 #include <algorithm>
 #include <chrono>
@@ -17,6 +19,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -25,40 +28,6 @@ namespace ia {
 
 
 namespace {
-
-struct DevError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-#define IA_CHECK(x)                                                                   \
-    do {                                                                              \
-        hipError_t e_ = (x);                                                          \
-        if (e_ != hipSuccess)                                                         \
-            throw DevError(std::string(#x) + ": " + hipGetErrorString(e_));           \
-    } while (0)
-
-// Grow-only device buffer.
-struct DBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    template <class T>
-    T *get(size_t n) {
-        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-        if (bytes > cap) {
-            if (p) IA_CHECK(hipFree(p));
-            p = nullptr;
-            const size_t want = std::max(bytes, cap + cap / 2);
-            IA_CHECK(hipMalloc(&p, want));
-            cap = want;
-        }
-        return (T *)p;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
 
 Hit identity_hit(double s) {
     Hit h;
@@ -150,9 +119,29 @@ class Engine {
     // Search the resident batch; out has one slot per loaded pair.
     // method 0: plain FP64 (bit-identical to the reference's arithmetic);
     // method 1: fast (FP32 NW fill, closed-form FP64 rotations).
+    // method 2: learned (method 1 + only the ranker's top-m local-fragment
+    //           work items of each pair are refined).
     int search(const NativeConfig *cfg, bool serial_bank, NativePairResult *out, int method = 0);
     int loaded() const { return n_loaded_; }
     double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // Running best score of every loaded pair after each of kStages stages
+    // (NaN: the stage did not run), row-major [pair][stage].
+    static constexpr int kStages = 10;
+    std::vector<double> stage_scores;
+    // Work-item ranker used by method 2 and its per-pair budget.
+    Mlp ranker;
+    int top_m = 64;
+    double fallback_margin = -1.0;  // < 0: never fall back to every work item
+    // Record sink: when set, every local-fragment work item of the search
+    // appends its descriptor row, its refined score (best of its two
+    // chains) and its pair index, up to cap rows.
+    struct RecordSink {
+        float *X, *y;
+        int *pair;
+        long long cap, n;
+    };
+    RecordSink *rec = nullptr;
+    hipStream_t stream() const { return s_; }
 
   private:
     hipStream_t s_;
@@ -161,6 +150,7 @@ class Engine {
     const NativeConfig *cfg_ = nullptr;
     bool serial_ = true;
     bool fast_ = false;
+    bool learned_ = false;
     std::vector<HostPair> hp_;
     std::vector<Hit> best_;
     std::vector<int> live_;  // pairs still in the pipeline (index into hp_)
@@ -206,7 +196,8 @@ class Engine {
     DBuf b_seeds, b_masks, b_banks, b_blk_bank, b_blk_first, b_cands, b_bank_out, b_lists_p,
         b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
         b_red_cnt, b_red_out, b_rs_pair, b_rs_in, b_rs_out, b_pairs, b_aoff, b_acnt, b_ascr,
-        b_soff, b_fofs, b_fcnt, b_moff;
+        b_soff, b_fofs, b_fcnt, b_moff, b_feat, b_fgroups, b_pred, b_isel, b_srow, b_sitems,
+        b_ssel, b_scnt, b_itemy, b_pmax;
 
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
@@ -216,7 +207,8 @@ class Engine {
                 &b_bank_out, &b_lists_p, &b_lists_n, &b_list_na, &b_lout, &b_groups, &b_xf,
                 &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
                 &b_rs_pair, &b_rs_in, &b_rs_out, &b_pairs, &b_aoff, &b_acnt, &b_ascr, &b_soff,
-                &b_fofs, &b_fcnt, &b_moff};
+                &b_fofs, &b_fcnt, &b_moff, &b_feat, &b_fgroups, &b_pred, &b_isel, &b_srow,
+                &b_sitems, &b_ssel, &b_scnt, &b_itemy, &b_pmax};
     }
 
     template <class T>
@@ -619,6 +611,14 @@ std::vector<Hit> Engine::dp_refine(const std::vector<int> &pairs, const std::vec
 
 
 void Engine::profile(const char *stage, std::chrono::steady_clock::time_point t0) {
+    static const char *names[kStages] = {"seed_bank",   "ss",           "gapless_threading",
+                                         "local_fragment", "threading_dp", "dense_local_fragment",
+                                         "topk_seed",   "dp_refine",    "offset_dp",
+                                         "weighted_refine"};
+    for (int k = 0; k < kStages; k++)
+        if (std::strcmp(stage, names[k]) == 0)
+            for (const HostPair &h : hp_)
+                stage_scores[(size_t)h.src * kStages + k] = best_[&h - hp_.data()].score;
     static const bool on = std::getenv("INDIALIGN_PROFILE") != nullptr;
     if (!on) return;
     const char *tag = std::getenv("INDIALIGN_PROFILE_TAG");
@@ -826,6 +826,7 @@ void Engine::stage_rescue() {
             if (best_[a].score < (dense ? rs2 : rs1)) P.push_back(a);
         if (P.empty()) return;
         std::vector<ChainGroup> groups;
+        std::vector<FeatGroup> fgs;
         std::vector<long long> offs;
         std::vector<int> cnts;
         long long base = 0;
@@ -857,11 +858,101 @@ void Engine::stage_rescue() {
             g.out_stride = 2;
             offs.push_back(base);
             cnts.push_back(2 * items);
-            if (items > 0) groups.push_back(g);
+            if (items > 0) {
+                groups.push_back(g);
+                FeatGroup f{};
+                f.pair = a;
+                f.items = items;
+                f.n_lens = g.n_lens;
+                f.jp = g.jp;
+                f.jn = g.jn;
+                std::memcpy(f.lens, g.lens, sizeof(f.lens));
+                std::memcpy(f.items_prefix, g.items_prefix, sizeof(f.items_prefix));
+                std::memcpy(f.n_ns, g.n_ns, sizeof(f.n_ns));
+                f.row0 = base / 2;
+                f.prior = (float)best_[a].score;
+                f.dense = dense ? 1.f : 0.f;
+                fgs.push_back(f);
+            }
             base += 2LL * items;
+        }
+        const long long rows = base / 2;
+        const bool rank = learned_ && !groups.empty();
+        float *X = nullptr;
+        if ((rank || rec) && rows > 0) {
+            X = b_feat.get<float>((size_t)rows * kFeat);
+            const FeatGroup *fdev = upload(b_fgroups, fgs);
+            launch_frag_features(s_, A_, fdev, (int)fgs.size(), rows, X);
+            launched();
+        }
+        if (rank) {
+            // refine only the ranker's top-m work items of every pair
+            float *pred = b_pred.get<float>(rows);
+            ranker.forward(s_, X, rows, pred);
+            std::vector<long long> row0(fgs.size()), sel_off(fgs.size());
+            std::vector<int> items(fgs.size());
+            long long ns = 0;
+            for (size_t k = 0; k < fgs.size(); k++) {
+                row0[k] = fgs[k].row0;
+                items[k] = fgs[k].items;
+                sel_off[k] = ns;
+                ns += std::min(fgs[k].items, top_m);
+            }
+            int *isel = b_isel.get<int>(ns);
+            float *pmax_d = b_pmax.get<float>(fgs.size());
+            launch_select_top(s_, pred, upload(b_srow, row0), upload(b_sitems, items),
+                              (int)fgs.size(), top_m, isel, upload(b_ssel, sel_off),
+                              b_scnt.get<int>(fgs.size()), pmax_d);
+            launched();
+            // fallback: a pair whose best predicted refined score clears its
+            // running best by the margin refines every work item
+            std::vector<float> pmax;
+            if (fallback_margin >= 0) pmax = download(pmax_d, fgs.size());
+            A_.isel = isel;
+            offs.clear();
+            cnts.clear();
+            size_t k = 0;
+            long long ob = 0;
+            for (int a : P) {
+                if (k < groups.size() && groups[k].pair == a) {
+                    ChainGroup &g = groups[k];
+                    const bool full = fallback_margin >= 0 &&
+                                      (double)pmax[k] >= best_[a].score + fallback_margin;
+                    const int c = full ? fgs[k].items : std::min(fgs[k].items, top_m);
+                    g.sel = full ? 0 : 1;
+                    g.sel_off = sel_off[k];
+                    g.n_chains = 2 * c;
+                    g.out_off = ob;
+                    offs.push_back(ob);
+                    cnts.push_back(2 * c);
+                    ob += 2LL * c;
+                    stats[7] += full ? 1 : 0;
+                    k++;
+                } else {
+                    offs.push_back(ob);
+                    cnts.push_back(0);
+                }
+            }
+            base = ob;
         }
         Hit *dout = b_chain_out.get<Hit>(base);
         run_chains(groups, nullptr, dout);
+        if (rec && !rank && rows > 0) {
+            float *y = b_itemy.get<float>(rows);
+            launch_item_best(s_, dout, rows, y);
+            const long long take = std::min(rows, rec->cap - rec->n);
+            if (take > 0) {
+                IA_CHECK(hipMemcpyAsync(rec->X + rec->n * kFeat, X, sizeof(float) * take * kFeat,
+                                        hipMemcpyDeviceToHost, s_));
+                IA_CHECK(hipMemcpyAsync(rec->y + rec->n, y, sizeof(float) * take,
+                                        hipMemcpyDeviceToHost, s_));
+                IA_CHECK(hipStreamSynchronize(s_));
+                long long r = rec->n;
+                for (const FeatGroup &f : fgs)
+                    for (int i = 0; i < f.items && r < rec->n + take; i++) rec->pair[r++] = hp_[f.pair].src;
+                rec->n += take;
+            }
+        }
         const std::vector<Hit> red = group_reduce(offs, cnts, dout);
         const std::vector<Hit> rs = rescore(P, red);
         for (size_t k = 0; k < P.size(); k++) offer(P[k], rs[k]);
@@ -1128,7 +1219,11 @@ int Engine::search(const NativeConfig *cfg, bool serial_bank, NativePairResult *
     for (double &x : stats) x = 0;
     cfg_ = cfg;
     serial_ = serial_bank;
-    fast_ = method == 1;
+    fast_ = method == 1 || method == 2;
+    learned_ = method == 2;
+    if (learned_ && !ranker.loaded())
+        throw DevError("learned method requested but no ranker is loaded (indialign_set_ranker)");
+    stage_scores.assign((size_t)n_loaded_ * kStages, std::nan(""));
     live_.clear();
     best_.assign(hp_.size(), identity_hit(kNeg));
     // Lnorm <= 2 -> score 0, identity (indialign.cpp:47-51)
@@ -1164,17 +1259,63 @@ namespace {
 std::mutex g_mu;
 std::unique_ptr<ia::Engine> g_engine;
 double g_stats[8];
+// Ranker parameters installed by indialign_set_ranker (host copy, so that an
+// engine re-created after a device error gets them again).
+std::vector<float> g_ranker;
+int g_top_m = 64;
+double g_margin = -1.0;
+unsigned g_ranker_ver = 0;
+unsigned g_engine_ranker_ver = 0;
+
+// With g_mu held and g_engine alive: install the current ranker.
+void sync_ranker() {
+    static ia::Engine *seen = nullptr;
+    if (seen != g_engine.get()) {
+        seen = g_engine.get();
+        g_engine_ranker_ver = 0;
+    }
+    if (g_engine_ranker_ver == g_ranker_ver) return;
+    if (g_ranker.empty()) g_engine->ranker.unload();
+    else g_engine->ranker.set_params(g_engine->stream(), g_ranker.data());
+    g_engine->top_m = g_top_m;
+    g_engine->fallback_margin = g_margin;
+    g_engine_ranker_ver = g_ranker_ver;
+}
+
+// Standalone ranker for training and direct evaluation (own stream).
+struct Trainer {
+    hipStream_t s = nullptr;
+    ia::Mlp mlp;
+    ia::DBuf X, t, perm, Xb, tb, y, losses;
+    Trainer() {
+        int dev = 0;
+        IA_CHECK(hipGetDevice(&dev));
+        IA_CHECK(hipStreamCreate(&s));
+    }
+};
+std::mutex g_tmu;
+std::unique_ptr<Trainer> g_trainer;
+
+Trainer &trainer() {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        throw ia::DevError("no HIP device available");
+    if (!g_trainer) g_trainer.reset(new Trainer());
+    return *g_trainer;
+}
 
 int run_search(int n, const NativePairInput *inp, const NativeConfig *cfg, NativePairResult *out,
                int serial, int method = 0) {
     const char *env = std::getenv("INDIALIGN_SERIAL_BANK");
     if (env && *env) serial = std::atoi(env) != 0;
     std::lock_guard<std::mutex> lk(g_mu);
+    if (method == 2 && g_ranker.empty()) return -4;
     try {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
             throw ia::DevError("no HIP device available");
         if (!g_engine) g_engine.reset(new ia::Engine());
+        sync_ranker();
         g_engine->load(inp, n);
         const int rc = g_engine->search(cfg, serial != 0, out, method);
         std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
@@ -1216,7 +1357,7 @@ int indialign_tmscore_search_batch(int n_pairs, const NativePairInput *inputs,
 int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
                         NativePairResult *outputs, int serial_bank, int method) {
     if (n_pairs <= 0) return 0;
-    if (method != 0 && method != 1) return -3;
+    if (method < 0 || method > 2) return -3;
     return run_search(n_pairs, inputs, cfg, outputs, serial_bank, method);
 }
 
@@ -1240,8 +1381,10 @@ int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int 
                         int method) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_engine) return -2;
-    if (method != 0 && method != 1) return -3;
+    if (method < 0 || method > 2) return -3;
+    if (method == 2 && g_ranker.empty()) return -4;
     try {
+        sync_ranker();
         const int rc = g_engine->search(cfg, serial_bank != 0, outputs, method);
         std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
         return rc;
@@ -1263,6 +1406,155 @@ int indialign_last_stats(double *out, int n) {
     const int m = n < 8 ? n : 8;
     for (int k = 0; k < m; k++) out[k] = g_stats[k];
     return m;
+}
+
+int indialign_last_stage_scores(double *out, int n_pairs) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_engine || n_pairs <= 0) return 0;
+    const int K = ia::Engine::kStages;
+    const int m = std::min<long long>(n_pairs, (long long)g_engine->stage_scores.size() / K);
+    std::memcpy(out, g_engine->stage_scores.data(), sizeof(double) * (size_t)m * K);
+    return m;
+}
+
+int indialign_set_ranker(const float *params, int top_m, double fallback_margin) {
+    if (params && top_m < 1) return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (params) g_ranker.assign(params, params + ia::kNParam);
+    else g_ranker.clear();
+    g_top_m = params ? top_m : 64;
+    g_margin = fallback_margin;
+    g_ranker_ver++;
+    return 0;
+}
+
+int indialign_collect_records(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
+                              NativePairResult *outputs, float *X, float *y, int *pair_idx,
+                              long long cap, long long *n_records) {
+    if (n_pairs <= 0 || cap < 0) return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    try {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+            throw ia::DevError("no HIP device available");
+        if (!g_engine) g_engine.reset(new ia::Engine());
+        ia::Engine::RecordSink sink{X, y, pair_idx, cap, 0};
+        g_engine->rec = &sink;
+        g_engine->load(inputs, n_pairs);
+        int rc = 0;
+        try {
+            rc = g_engine->search(cfg, true, outputs, 0);
+        } catch (...) {
+            g_engine->rec = nullptr;
+            throw;
+        }
+        g_engine->rec = nullptr;
+        std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
+        *n_records = sink.n;
+        return rc;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "indialign_amd: %s\n", e.what());
+        g_engine.reset();
+        return -2;
+    }
+}
+
+int indialign_mlp_forward(const float *params, const float *X, long long n, float *y) {
+    if (n < 0) return -1;
+    if (n == 0) return 0;
+    std::lock_guard<std::mutex> lk(g_tmu);
+    try {
+        Trainer &T = trainer();
+        T.mlp.set_params(T.s, params);
+        float *dX = T.X.get<float>((size_t)n * ia::kFeat);
+        float *dy = T.y.get<float>((size_t)n);
+        IA_CHECK(hipMemcpyAsync(dX, X, sizeof(float) * n * ia::kFeat, hipMemcpyHostToDevice, T.s));
+        T.mlp.forward(T.s, dX, n, dy);
+        IA_CHECK(hipMemcpyAsync(y, dy, sizeof(float) * n, hipMemcpyDeviceToHost, T.s));
+        IA_CHECK(hipStreamSynchronize(T.s));
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "indialign_amd: %s\n", e.what());
+        g_trainer.reset();
+        return -2;
+    }
+}
+
+int indialign_mlp_grad(const float *params, const float *X, const float *t, int n, float *grad,
+                       float *loss) {
+    if (n <= 0 || n > ia::Mlp::kChunk) return -1;
+    std::lock_guard<std::mutex> lk(g_tmu);
+    try {
+        Trainer &T = trainer();
+        T.mlp.set_params(T.s, params);
+        float *dX = T.X.get<float>((size_t)n * ia::kFeat);
+        float *dt = T.t.get<float>((size_t)n);
+        IA_CHECK(hipMemcpyAsync(dX, X, sizeof(float) * (size_t)n * ia::kFeat,
+                                hipMemcpyHostToDevice, T.s));
+        IA_CHECK(hipMemcpyAsync(dt, t, sizeof(float) * n, hipMemcpyHostToDevice, T.s));
+        T.mlp.backward(T.s, dX, dt, n);
+        IA_CHECK(hipMemcpyAsync(grad, T.mlp.grad(), sizeof(float) * ia::kNParam,
+                                hipMemcpyDeviceToHost, T.s));
+        IA_CHECK(hipMemcpyAsync(loss, T.mlp.loss_dev(), sizeof(float), hipMemcpyDeviceToHost, T.s));
+        IA_CHECK(hipStreamSynchronize(T.s));
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "indialign_amd: %s\n", e.what());
+        g_trainer.reset();
+        return -2;
+    }
+}
+
+int indialign_mlp_train(float *params, const float *X, const float *t, long long n, int epochs,
+                        int batch, float lr, unsigned seed, float *epoch_loss) {
+    if (n <= 0 || epochs < 0 || batch <= 0 || batch > ia::Mlp::kChunk) return -1;
+    std::lock_guard<std::mutex> lk(g_tmu);
+    try {
+        Trainer &T = trainer();
+        T.mlp.set_params(T.s, params);
+        T.mlp.reset_adam(T.s);
+        float *dX = T.X.get<float>((size_t)n * ia::kFeat);
+        float *dt = T.t.get<float>((size_t)n);
+        IA_CHECK(hipMemcpyAsync(dX, X, sizeof(float) * n * ia::kFeat, hipMemcpyHostToDevice, T.s));
+        IA_CHECK(hipMemcpyAsync(dt, t, sizeof(float) * n, hipMemcpyHostToDevice, T.s));
+        float *Xb = T.Xb.get<float>((size_t)batch * ia::kFeat);
+        float *tb = T.tb.get<float>((size_t)batch);
+        const long long nb = (n + batch - 1) / batch;
+        float *losses = T.losses.get<float>((size_t)nb);
+        std::vector<int> perm(n);
+        std::vector<float> lh(nb);
+        for (int e = 0; e < epochs; e++) {
+            for (long long i = 0; i < n; i++) perm[i] = (int)i;
+            std::mt19937 rng(seed + 7919u * (unsigned)e);
+            for (long long i = n - 1; i > 0; i--) {  // Fisher-Yates
+                const long long j = (long long)(rng() % (unsigned long long)(i + 1));
+                std::swap(perm[i], perm[j]);
+            }
+            int *dperm = T.perm.get<int>((size_t)n);
+            IA_CHECK(hipMemcpyAsync(dperm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                                    T.s));
+            for (long long b = 0; b < nb; b++) {
+                const int m = (int)std::min<long long>(batch, n - b * batch);
+                ia::launch_gather_rows(T.s, dX, dt, dperm + b * batch, m, Xb, tb);
+                T.mlp.backward(T.s, Xb, tb, m);
+                IA_CHECK(hipMemcpyAsync(losses + b, T.mlp.loss_dev(), sizeof(float),
+                                        hipMemcpyDeviceToDevice, T.s));
+                T.mlp.adam_step(T.s, lr);
+            }
+            IA_CHECK(hipMemcpyAsync(lh.data(), losses, sizeof(float) * nb, hipMemcpyDeviceToHost,
+                                    T.s));
+            IA_CHECK(hipStreamSynchronize(T.s));
+            double acc = 0;
+            for (float v : lh) acc += v;
+            if (epoch_loss) epoch_loss[e] = (float)(acc / (double)nb);
+        }
+        T.mlp.get_params(T.s, params);
+        return 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "indialign_amd: %s\n", e.what());
+        g_trainer.reset();
+        return -2;
+    }
 }
 
 }  // extern "C"

@@ -45,6 +45,7 @@ struct Arena {
     const float *qfx, *qfy, *qfz;                      // compact native, FP32 (fast method)
     const char *ssp, *ssn;                             // secondary structure (full index)
     const PairInfo *info;
+    const int *isel;                                   // selected work items (learned method)
 };
 
 IA_HD void set_identity(double *R, double *t) {

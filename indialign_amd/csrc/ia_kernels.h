@@ -43,7 +43,9 @@ enum ScoreMode { SM_TM = 0, SM_SS = 1, SM_IA4 = 2 };
 //              xf[xf_off + i*xf_stride + (xf_per_chain ? g : 0)].
 //   CI_FRAG:   local-fragment rescue work items (rescue.h:69-78): item
 //              index -> (flen, ps, ns) over compact indices; start = Kabsch
-//              of the fragment pair (rescue.h:84-91).
+//              of the fragment pair (rescue.h:84-91).  With sel != 0 the
+//              chain's item is A.isel[sel_off + i] (a selected subset, in
+//              increasing item order) and results stay at index i.
 //   CI_OFFSET: offset k = k0 + (c/2)*kstep; start = Kabsch of the pairing
 //              pred[i] <-> native[i-k] (indialign.cpp:234-247); k == 0 or
 //              fewer than 3 pairs -> the chain reports "skipped".
@@ -67,6 +69,8 @@ struct ChainGroup {
     int n_ns[9];
     double sd8;
     double gap_ss;        // gap used by SM_SS / SM_IA4 chains (ss.h:75,144)
+    int sel;              // CI_FRAG: 0 = every item, 1 = the items A.isel[sel_off + i]
+    long long sel_off;
 };
 
 // Per-chain extra outputs of CM_LIST chains.
@@ -102,6 +106,66 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
                    int n_slots, bool cols_in_lds, bool fast);
+// ---- learned ranking of local-fragment work items ------------------------
+constexpr int kFeat = 96;   // descriptor width
+constexpr int kHid = 256;   // hidden units per layer
+// MLP parameters, one FP32 array: W1[kHid][kFeat], b1[kHid], W2[kHid][kHid],
+// b2[kHid], w3[kHid], b3.
+constexpr long long kOffW1 = 0;
+constexpr long long kOffB1 = kOffW1 + (long long)kHid * kFeat;
+constexpr long long kOffW2 = kOffB1 + kHid;
+constexpr long long kOffB2 = kOffW2 + (long long)kHid * kHid;
+constexpr long long kOffW3 = kOffB2 + kHid;
+constexpr long long kOffB3 = kOffW3 + kHid;
+constexpr long long kNParam = kOffB3 + 1;
+
+// The local-fragment work items of one pair (the CI_FRAG enumeration of its
+// chain group) occupy rows [row0, row0 + items) of the feature matrix.
+struct FeatGroup {
+    int pair, items, n_lens, jp, jn;
+    int lens[9];
+    int items_prefix[10];
+    int n_ns[9];
+    long long row0;
+    float prior;   // running best score of the pair before the stage
+    float dense;   // 1 for the dense pass, 0 for the first pass
+};
+void launch_frag_features(hipStream_t s, const Arena &A, const FeatGroup *groups, int n_groups,
+                          long long n_rows, float *X);
+
+// C(m,n) = epi(sum_k A(m,k) B(k,n)) with A(m,k) = A[m*sam + k*sak],
+// B(k,n) = B[k*sbk + n*sbn], C(m,n) = C[m*scm + n*scn]; operands rounded to
+// BF16, products accumulated in FP32 on the matrix cores.
+enum GemmEpi { EPI_STORE = 0, EPI_BIAS_RELU = 1, EPI_BIAS = 2, EPI_MASK = 3 };
+struct GemmArgs {
+    const float *A, *B;
+    float *C;
+    long long sam, sak, sbk, sbn, scm, scn;
+    int M, N, K, epi;
+    const float *bias;          // EPI_BIAS*: bias[n]
+    const float *mask;          // EPI_MASK: C *= (mask(m,n) > 0), same strides as C
+};
+void launch_gemm(hipStream_t s, const GemmArgs &g);
+// dh2(m,n) = dy[m] * w3[n] * (h2(m,n) > 0)
+void launch_head_back(hipStream_t s, const float *dy, const float *w3, const float *h2, int n,
+                      float *dh2);
+// dy[m] = 2 (y[m] - t[m]) / n and loss = mean (y - t)^2 (FP32, one value)
+void launch_mse(hipStream_t s, const float *y, const float *t, int n, float *dy, float *loss);
+void launch_adam(hipStream_t s, float *p, const float *g, float *m, float *v, long long n,
+                 float lr, float b1, float b2, float eps, int step);
+// Per group: indices (relative to the group) of the top-m predictions, ties
+// to the lower index, written in increasing index order to
+// sel[sel_off[g] ...]; count[g] = min(m, items); pmax[g] = largest prediction.
+void launch_select_top(hipStream_t s, const float *pred, const long long *row0,
+                       const int *items, int n_groups, int m, int *sel, const long long *sel_off,
+                       int *count, float *pmax);
+
+// y[i] = best score of the two chains of work item i (out[2i], out[2i+1]).
+void launch_item_best(hipStream_t s, const Hit *out, long long n_items, float *y);
+// dst[r] = src[idx[r]] (rows of kFeat values), tdst[r] = tsrc[idx[r]]
+void launch_gather_rows(hipStream_t s, const float *src, const float *tsrc, const int *idx,
+                        int n, float *dst, float *tdst);
+
 size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds);
 int chain_blocks_per_cu(size_t lds);
 

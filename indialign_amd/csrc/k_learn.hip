@@ -1,0 +1,441 @@
+// Learned ranking of the local-fragment rescue work items (rescue.h:60-118).
+//
+// The dense local-fragment pass refines every fragment pair (flen, ps, ns) of
+// a pair by two NW chains; almost all of them end below the pair's best.  The
+// learned method scores every work item with a small MLP over a 96-value
+// descriptor and refines only the top-m items of each pair.  This file holds
+//   - the descriptor kernel (one wavefront per work item),
+//   - a BF16-operand / FP32-accumulate GEMM on the matrix cores
+//     (v_mfma_f32_32x32x8_bf16) that the MLP's forward and backward passes
+//     are made of, plus the small elementwise kernels of training (MSE, head
+//     backward, Adam),
+//   - the per-pair top-m selection (radix select, ties to the lower index).
+#include "ia_kernels.h"
+#include "wave.h"
+
+#include <math.h>
+
+namespace ia {
+
+namespace {
+
+// ---- work-item descriptor ---------------------------------------------------
+//
+// Row layout (kFeat = 96 values, all O(1)):
+//   0 flen/ml   1 flen/100   2 ps/plen   3 ns/nlen   4 (ps-ns)/ml
+//   5 [ps == ns]   6 Lnorm/300   7 d0/10   8 dense pass   9 prior best score
+//  10 fragment RMSD/d0 (<= 10)   11 TM-score of the fragment superposition,
+//     same-index pairing   12 same, register pairing pred a <-> native a+ns-ps
+//  13..23 fraction of same-index residues within the pair's d0 search cutoffs
+//  24 C/11   25 plen/300   26 nlen/300   27 sampled pairs/276   28 log(1+flen)/5
+//  29 register-pairing overlap / ml   30..31 zero
+//  32..95 histogram of |d_pred(i,j) - d_native(i',j')| over up to 24 evenly
+//     spaced fragment positions, 0.25 A bins (last bin open), sum 1.
+constexpr int kHistPos = 24;
+
+IA_D int find_feat_group(const FeatGroup *g, int n, long long row) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g[mid].row0 <= row) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void frag_feature_kernel(Arena A, const FeatGroup *groups,
+                                                           int n_groups, long long n_rows,
+                                                           float *X) {
+    __shared__ unsigned hist_s[4][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned *hist = hist_s[wv];
+    const long long stride = (long long)gridDim.x * 4;
+    for (long long row = (long long)blockIdx.x * 4 + wv; row < n_rows; row += stride) {
+        const FeatGroup &G = groups[find_feat_group(groups, n_groups, row)];
+        const PairInfo &pi = A.info[G.pair];
+        const int off = pi.off;
+        const int item = (int)(row - G.row0);
+        int L_ = 0;
+        while (L_ + 1 < G.n_lens && G.items_prefix[L_ + 1] <= item) L_++;
+        const int r = item - G.items_prefix[L_];
+        const int flen = G.lens[L_];
+        const int ps = (r / G.n_ns[L_]) * G.jp, ns = (r % G.n_ns[L_]) * G.jn;
+        double R[9], t[3];
+        wave_kabsch<true>(flen,
+                          [&](int q, double &x, double &y, double &z, double &a, double &b,
+                              double &c) {
+                              x = A.pcx[off + ps + q]; y = A.pcy[off + ps + q];
+                              z = A.pcz[off + ps + q];
+                              a = A.qcx[off + ns + q]; b = A.qcy[off + ns + q];
+                              c = A.qcz[off + ns + q];
+                          },
+                          R, t);
+        // fragment RMSD
+        double e = 0.0;
+        for (int q = lane; q < flen; q += 64)
+            e += dist2_moved(R, t, A.pcx[off + ps + q], A.pcy[off + ps + q], A.pcz[off + ps + q],
+                             A.qcx[off + ns + q], A.qcy[off + ns + q], A.qcz[off + ns + q]);
+        e = wave_sum(e);
+        // same-index TM-score and cutoff counts
+        const double d0sq = pi.d0 * pi.d0;
+        double tm = 0.0;
+        float cnt[kMaxCut];
+#pragma unroll
+        for (int c = 0; c < kMaxCut; c++) cnt[c] = 0.f;
+        for (int i = lane; i < pi.N; i += 64) {
+            if (!A.v[off + i]) continue;
+            const double d2 = dist2_moved(R, t, A.px[off + i], A.py[off + i], A.pz[off + i],
+                                          A.qx[off + i], A.qy[off + i], A.qz[off + i]);
+            tm += 1.0 / (1.0 + d2 / d0sq);
+#pragma unroll
+            for (int c = 0; c < kMaxCut; c++)
+                cnt[c] += (c < pi.C && d2 < pi.cut[c] * pi.cut[c]) ? 1.f : 0.f;
+        }
+        tm = wave_sum(tm);
+        // register pairing a <-> a + (ns - ps) over the compact rows
+        const int sh = ns - ps;
+        double tmr = 0.0;
+        int ov = 0;
+        for (int a = lane; a < pi.plen; a += 64) {
+            const int b = a + sh;
+            if (b < 0 || b >= pi.nlen) continue;
+            ov++;
+            const double d2 = dist2_moved(R, t, A.pcx[off + a], A.pcy[off + a], A.pcz[off + a],
+                                          A.qcx[off + b], A.qcy[off + b], A.qcz[off + b]);
+            tmr += 1.0 / (1.0 + d2 / d0sq);
+        }
+        tmr = wave_sum(tmr);
+        for (int m = 32; m > 0; m >>= 1) ov += __shfl_xor(ov, m, 64);
+        // intra-fragment distance-difference histogram
+        hist[lane] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const int m = flen < kHistPos ? flen : kHistPos;
+        for (int k1 = 0; k1 + 1 < m; k1++) {
+            const int k2 = k1 + 1 + lane;
+            if (k2 >= m) continue;
+            const int q1 = (k1 * (flen - 1)) / (m - 1), q2 = (k2 * (flen - 1)) / (m - 1);
+            const double ax = A.pcx[off + ps + q1] - A.pcx[off + ps + q2];
+            const double ay = A.pcy[off + ps + q1] - A.pcy[off + ps + q2];
+            const double az = A.pcz[off + ps + q1] - A.pcz[off + ps + q2];
+            const double bx = A.qcx[off + ns + q1] - A.qcx[off + ns + q2];
+            const double by = A.qcy[off + ns + q1] - A.qcy[off + ns + q2];
+            const double bz = A.qcz[off + ns + q1] - A.qcz[off + ns + q2];
+            const double dd = fabs(sqrt(ax * ax + ay * ay + az * az) -
+                                   sqrt(bx * bx + by * by + bz * bz));
+            const int bin = dd >= 15.75 ? 63 : (int)(dd * 4.0);
+            atomicAdd(&hist[bin], 1u);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        const int np = m * (m - 1) / 2;
+        const int ml = pi.plen < pi.nlen ? pi.plen : pi.nlen;
+        float tot[kMaxCut];
+#pragma unroll
+        for (int c = 0; c < kMaxCut; c++) {
+            float v = cnt[c];
+            for (int mm = 32; mm > 0; mm >>= 1) v += __shfl_xor(v, mm, 64);
+            tot[c] = v;
+        }
+        float *x = X + row * kFeat;
+        x[32 + lane] = np > 0 ? (float)hist[lane] / (float)np : 0.f;
+        if (lane < 32) {
+            float f = 0.f;
+            const double L = pi.Lnorm;
+            switch (lane) {
+                case 0: f = (float)flen / (float)ml; break;
+                case 1: f = (float)flen / 100.f; break;
+                case 2: f = (float)ps / (float)pi.plen; break;
+                case 3: f = (float)ns / (float)pi.nlen; break;
+                case 4: f = (float)(ps - ns) / (float)ml; break;
+                case 5: f = ps == ns ? 1.f : 0.f; break;
+                case 6: f = (float)(L / 300.0); break;
+                case 7: f = (float)(pi.d0 / 10.0); break;
+                case 8: f = G.dense; break;
+                case 9: f = G.prior; break;
+                case 10: f = (float)fmin(sqrt(e / flen) / pi.d0, 10.0); break;
+                case 11: f = (float)(tm / L); break;
+                case 12: f = (float)(tmr / L); break;
+                case 24: f = (float)pi.C / 11.f; break;
+                case 25: f = (float)pi.plen / 300.f; break;
+                case 26: f = (float)pi.nlen / 300.f; break;
+                case 27: f = (float)np / 276.f; break;
+                case 28: f = logf(1.f + (float)flen) / 5.f; break;
+                case 29: f = (float)ov / (float)ml; break;
+                default: break;
+            }
+#pragma unroll
+            for (int c = 0; c < kMaxCut; c++)
+                if (lane == 13 + c) f = (float)(tot[c] / L);
+            x[lane] = f;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ---- BF16 GEMM on the matrix cores --------------------------------------------
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+IA_D short bf16_rne(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+// One workgroup = 4 wavefronts = a 64 x 64 tile of C; each wavefront owns a
+// 32 x 32 sub-tile and runs v_mfma_f32_32x32x8_bf16 over K in steps of 8.
+// Operand layout of the instruction: lane l holds A(row l%32, k 4*(l/32)+j)
+// and B(k 4*(l/32)+j, col l%32), j = 0..3; result v of lane l is
+// C(8*(v/4) + 4*(l/32) + v%4, l%32).
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long m0 = (long long)blockIdx.y * 64 + (wv >> 1) * 32;
+    const int n0 = blockIdx.x * 64 + (wv & 1) * 32;
+    const int r = lane & 31, kq = (lane >> 5) * 4;
+    const long long am = m0 + r;
+    const int bn = n0 + r;
+    const bool a_ok = am < g.M, b_ok = bn < g.N;
+    const float *Ap = g.A + (a_ok ? am * g.sam : 0);
+    const float *Bp = g.B + (b_ok ? (long long)bn * g.sbn : 0);
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[v] = 0.f;
+    for (int k0 = 0; k0 < g.K; k0 += 8) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = k0 + kq + j;
+            const bool kin = k < g.K;
+            const float av = (a_ok && kin) ? Ap[(long long)k * g.sak] : 0.f;
+            const float bv = (b_ok && kin) ? Bp[(long long)k * g.sbk] : 0.f;
+            a[j] = bf16_rne(av);
+            b[j] = bf16_rne(bv);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const int n = n0 + r;
+    if (n >= g.N) return;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const long long m = m0 + 8 * (v >> 2) + (lane >> 5) * 4 + (v & 3);
+        if (m >= g.M) continue;
+        float x = acc[v];
+        const long long ci = m * g.scm + (long long)n * g.scn;
+        if (g.epi == EPI_BIAS_RELU) x = fmaxf(x + g.bias[n], 0.f);
+        else if (g.epi == EPI_BIAS) x = x + g.bias[n];
+        else if (g.epi == EPI_MASK) x = g.mask[ci] > 0.f ? x : 0.f;
+        g.C[ci] = x;
+    }
+}
+
+__global__ void head_back_kernel(const float *dy, const float *w3, const float *h2, long long n,
+                                 float *dh2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * kHid) return;
+    const long long m = i / kHid;
+    const int j = (int)(i - m * kHid);
+    dh2[i] = h2[i] > 0.f ? dy[m] * w3[j] : 0.f;
+}
+
+// One workgroup: dy and the mean squared error (FP32, fixed reduction order).
+__global__ __launch_bounds__(1024) void mse_kernel(const float *y, const float *t, int n,
+                                                   float *dy, float *loss) {
+    __shared__ float part[1024];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const float d = y[i] - t[i];
+        dy[i] = 2.f * d / (float)n;
+        s += d * d;
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[0] = part[0] / (float)n;
+}
+
+__global__ void adam_kernel(float *p, const float *g, float *m, float *v, long long n, float lr,
+                            float b1, float b2, float eps, float c1, float c2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---- per-pair top-m selection ---------------------------------------------------
+IA_D unsigned order_key(float f) {
+    const unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Block-wide exclusive prefix count of a flag (256 threads, 4 wavefronts).
+IA_D int block_excl(bool flag, int *wtot, int &total) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned long long b = __ballot(flag);
+    const int below = __popcll(b & ((1ull << lane) - 1ull));
+    __syncthreads();
+    if (lane == 0) wtot[wv] = __popcll(b);
+    __syncthreads();
+    int base = 0;
+    total = 0;
+    for (int w = 0; w < 4; w++) {
+        if (w < wv) base += wtot[w];
+        total += wtot[w];
+    }
+    return base + below;
+}
+
+__global__ __launch_bounds__(256) void select_top_kernel(const float *pred, const long long *row0,
+                                                         const int *items, int m, int *sel,
+                                                         const long long *sel_off, int *count,
+                                                         float *pmax) {
+    __shared__ unsigned hist[256];
+    __shared__ unsigned s_prefix, s_mask, s_need;
+    __shared__ int wtot[4];
+    __shared__ float wmax[4];
+    const int gi = blockIdx.x;
+    const int n = items[gi];
+    const float *pv = pred + row0[gi];
+    int *out = sel + sel_off[gi];
+    {   // largest prediction of the group
+        float mx = -INFINITY;
+        for (int i = threadIdx.x; i < n; i += 256) mx = fmaxf(mx, pv[i]);
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            pmax[gi] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    }
+    if (n <= m) {
+        for (int i = threadIdx.x; i < n; i += 256) out[i] = i;
+        if (threadIdx.x == 0) count[gi] = n;
+        return;
+    }
+    if (threadIdx.x == 0) { s_prefix = 0; s_mask = 0; s_need = (unsigned)m; }
+    for (int pass = 0; pass < 4; pass++) {
+        const int shift = 24 - 8 * pass;
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        const unsigned prefix = s_prefix, mask = s_mask;
+        for (int i = threadIdx.x; i < n; i += 256) {
+            const unsigned k = order_key(pv[i]);
+            if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned cum = 0, need = s_need;
+            for (int d = 255; d >= 0; d--) {
+                if (cum + hist[d] >= need) {
+                    s_prefix = prefix | ((unsigned)d << shift);
+                    s_mask = mask | (255u << shift);
+                    s_need = need - cum;
+                    break;
+                }
+                cum += hist[d];
+            }
+        }
+        __syncthreads();
+    }
+    const unsigned T = s_prefix;
+    const int need_eq = (int)s_need;
+    int written = 0, eq_seen = 0;
+    for (int c0 = 0; c0 < n; c0 += 256) {
+        const int i = c0 + threadIdx.x;
+        const unsigned k = i < n ? order_key(pv[i]) : 0u;
+        const bool gt = i < n && k > T, eq = i < n && k == T;
+        int eq_tot = 0;
+        const int eq_rank = eq_seen + block_excl(eq, wtot, eq_tot);
+        const bool take = gt || (eq && eq_rank < need_eq);
+        int take_tot = 0;
+        const int pos = written + block_excl(take, wtot, take_tot);
+        if (take) out[pos] = i;
+        written += take_tot;
+        eq_seen += eq_tot;
+    }
+    if (threadIdx.x == 0) count[gi] = m;
+}
+
+__global__ void gather_rows_kernel(const float *src, const float *tsrc, const int *idx, int n,
+                                   float *dst, float *tdst) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * kFeat) return;
+    const int r = (int)(i / kFeat), c = (int)(i - (long long)r * kFeat);
+    const long long s = (long long)idx[r];
+    dst[i] = src[s * kFeat + c];
+    if (c == 0) tdst[r] = tsrc[s];
+}
+
+__global__ void item_best_kernel(const Hit *out, long long n, float *y) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = out[2 * i].score, b = out[2 * i + 1].score;
+    y[i] = (float)(a > b ? a : b);
+}
+
+}  // namespace
+
+void launch_item_best(hipStream_t s, const Hit *out, long long n_items, float *y) {
+    if (n_items <= 0) return;
+    hipLaunchKernelGGL(item_best_kernel, dim3((unsigned)((n_items + 255) / 256)), dim3(256), 0, s,
+                       out, n_items, y);
+}
+
+void launch_frag_features(hipStream_t s, const Arena &A, const FeatGroup *groups, int n_groups,
+                          long long n_rows, float *X) {
+    if (n_rows <= 0) return;
+    long long blocks = (n_rows + 3) / 4;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    hipLaunchKernelGGL(frag_feature_kernel, dim3((unsigned)blocks), dim3(256), 0, s, A, groups,
+                       n_groups, n_rows, X);
+}
+
+void launch_gemm(hipStream_t s, const GemmArgs &g) {
+    if (g.M <= 0 || g.N <= 0) return;
+    const dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 63) / 64));
+    hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, g);
+}
+
+void launch_head_back(hipStream_t s, const float *dy, const float *w3, const float *h2, int n,
+                      float *dh2) {
+    const long long tot = (long long)n * kHid;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(head_back_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dy,
+                       w3, h2, (long long)n, dh2);
+}
+
+void launch_mse(hipStream_t s, const float *y, const float *t, int n, float *dy, float *loss) {
+    hipLaunchKernelGGL(mse_kernel, dim3(1), dim3(1024), 0, s, y, t, n, dy, loss);
+}
+
+void launch_adam(hipStream_t s, float *p, const float *g, float *m, float *v, long long n,
+                 float lr, float b1, float b2, float eps, int step) {
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, g, m,
+                       v, n, lr, b1, b2, eps, c1, c2);
+}
+
+void launch_select_top(hipStream_t s, const float *pred, const long long *row0,
+                       const int *items, int n_groups, int m, int *sel, const long long *sel_off,
+                       int *count, float *pmax) {
+    if (n_groups <= 0) return;
+    hipLaunchKernelGGL(select_top_kernel, dim3(n_groups), dim3(256), 0, s, pred, row0, items, m,
+                       sel, sel_off, count, pmax);
+}
+
+void launch_gather_rows(hipStream_t s, const float *src, const float *tsrc, const int *idx,
+                        int n, float *dst, float *tdst) {
+    const long long tot = (long long)n * kFeat;
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                       src, tsrc, idx, n, dst, tdst);
+}
+
+}  // namespace ia
