@@ -16,10 +16,11 @@ baseline is the reference library compiled from source (oracle/_ref), batch
 entry with OpenMP over pairs on the host cores, on a bounded sample of the
 same workload.
 
-The `learned` object times method 2 on the same batch (FP32 NW fill, and
+The `learned` object times method 2 on the same batch (the fast method, and
 of the local-fragment rescue work items only the top-64 per pair by the
-packaged trained ranker are refined) and gives its TM loss against the plain
-FP64 scores of the same pairs.
+packaged trained ranker are refined), the `fast` object method 1 (one step);
+both give their TM loss against the plain FP64 scores of the same pairs,
+overall and per pipeline stage.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs P]
 """
@@ -30,6 +31,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -62,32 +65,53 @@ def cpu_baseline(pairs, sample):
                       f"reference batch entry, {dt:.1f} s"}
 
 
-def learned_leg(ia, cfg, plain_scores, args, plain_s_per_step):
-    """Method 2 (FP32 NW fill + the packaged trained ranker, top-m
-    local-fragment work items per pair) on the same resident batch; TM loss
-    against the plain FP64 scores of the timed steps."""
-    ia.load_default_ranker(top_m=args.top_m)
-    for w in range(args.warmup):
+def stage_delta(ia, plain_stage, stage):
+    """max / mean |dTM| of the running best after each pipeline stage,
+    over pairs where the stage ran in both searches."""
+    out = {}
+    for k, name in enumerate(ia.STAGES):
+        a, b = plain_stage[:, k], stage[:, k]
+        ok = np.isfinite(a) & np.isfinite(b)
+        if ok.any():
+            d = np.abs(a[ok] - b[ok])
+            out[name] = {"max": float(d.max()), "mean": float(d.mean()), "pairs": int(ok.sum())}
+    return out
+
+
+def method_leg(ia, cfg, plain_scores, plain_stage, args, plain_s_per_step, method, steps, warmup):
+    """Another method on the same resident batch; TM loss against the plain
+    FP64 scores of the same pairs, overall and per stage."""
+    label = {ia.METHOD_FAST: "fast", ia.METHOD_LEARNED: "learned"}[method]
+    if method == ia.METHOD_LEARNED:
+        ia.load_default_ranker(top_m=args.top_m)
+    for w in range(warmup):
         t = time.perf_counter()
-        ia.batch_run(cfg, serial_bank=1, method=ia.METHOD_LEARNED)
-        log(f"[bench] learned warmup {w}: {time.perf_counter() - t:.2f} s")
+        ia.batch_run(cfg, serial_bank=1, method=method)
+        log(f"[bench] {label} warmup {w}: {time.perf_counter() - t:.2f} s")
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         t = time.perf_counter()
-        s, _, _ = ia.batch_run(cfg, serial_bank=1, method=ia.METHOD_LEARNED)
-        log(f"[bench] learned step {k}: {time.perf_counter() - t:.2f} s")
-    el = (time.perf_counter() - t0) / args.steps
+        s, _, _ = ia.batch_run(cfg, serial_bank=1, method=method)
+        log(f"[bench] {label} step {k}: {time.perf_counter() - t:.2f} s")
+    el = (time.perf_counter() - t0) / steps
     st = ia.last_stats()
+    stage = ia.last_stage_scores(len(s))
     ia.set_ranker(None)
     d = plain_scores - s
     n = len(s)
-    return {"method": f"FP32 NW fill + trained ranker, top-{args.top_m} local-fragment work items",
+    desc = {ia.METHOD_FAST: "FP32 NW fill, single-pass FP64 moments, closed-form quaternion rotation",
+            ia.METHOD_LEARNED: f"fast method + trained ranker, top-{args.top_m} local-fragment "
+                               "work items per pair and pass"}[method]
+    return {"method": desc,
             "value": round(n / el, 3), "unit": "pairs/s", "ms_per_step": round(1000 * el, 2),
+            "steps": steps,
             "speedup_vs_plain": round(plain_s_per_step / el, 3),
             "tm_loss_vs_plain": {"mean": float(d.mean()), "max": float(d.max()),
                                  "pairs_over_1e-3": int((d > 1e-3).sum()),
                                  "pairs_over_1e-4": int((d > 1e-4).sum())},
-            "nw_cells": st["nw_cells"], "chain_ms": round(st["chain_ms"], 2)}
+            "stage_abs_dtm_vs_plain": stage_delta(ia, plain_stage, stage),
+            "nw_cells": st["nw_cells"], "chain_ms": round(st["chain_ms"], 2),
+            "bank_ms": round(st["bank_ms"], 2)}
 
 
 def measured_traffic():
@@ -148,6 +172,7 @@ def main():
         t = time.perf_counter()
         scores, R, T = ia.batch_run(cfg, serial_bank=1)
         step_stats.append(ia.last_stats())
+        plain_stage = ia.last_stage_scores(len(pairs))
         log(f"[bench] step {k}: {time.perf_counter() - t:.2f} s")
     elapsed = time.perf_counter() - t_start
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -183,7 +208,10 @@ def main():
         "mean_score": round(float(scores.mean()), 6),
     }
     if not args.no_learned:
-        out["learned"] = learned_leg(ia, cfg, scores, args, elapsed / args.steps)
+        out["learned"] = method_leg(ia, cfg, scores, plain_stage, args, elapsed / args.steps,
+                                    ia.METHOD_LEARNED, args.steps, args.warmup)
+        out["fast"] = method_leg(ia, cfg, scores, plain_stage, args, elapsed / args.steps,
+                                 ia.METHOD_FAST, 1, 0)
     if not args.no_cpu:
         log(f"[bench] cpu baseline on {args.cpu_sample} pairs ...")
         out["cpu_baseline"] = cpu_baseline(pairs, args.cpu_sample)

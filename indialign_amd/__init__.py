@@ -342,7 +342,9 @@ def init_params(seed: int = 0) -> np.ndarray:
     o = w1 + 256
     p[o:o + w2] = rng.uniform(-1, 1, w2) * np.sqrt(6.0 / 256)
     o += w2 + 256
-    p[o:o + 256] = rng.uniform(-1, 1, 256) * np.sqrt(6.0 / 256)
+    # small output weights: the untrained ranker is close to its skip input
+    # (the refined same-index TM-score of each work item)
+    p[o:o + 256] = rng.uniform(-1, 1, 256) * 0.01
     return p
 
 

@@ -290,6 +290,25 @@ struct KabschSeq {
         solve_rotation<FAST>(h, R);
         finish_t(cp, cq, R, t);
     }
+    // Single-pass alternative (fast method): raw sums and second moments,
+    // H = sum x q^T - cnt cp cq^T after finish_moments().  In FP64 the
+    // cancellation costs ~|x||q| / |H| ulps, about 1e-15 relative for
+    // coordinates within 100 A.
+    IA_HD void add_moment(double x, double y, double z, double a, double b, double c) {
+        cp[0] += x; cp[1] += y; cp[2] += z;
+        cq[0] += a; cq[1] += b; cq[2] += c;
+        cnt += 1.0;
+        h[0] += x * a; h[1] += x * b; h[2] += x * c;
+        h[3] += y * a; h[4] += y * b; h[5] += y * c;
+        h[6] += z * a; h[7] += z * b; h[8] += z * c;
+    }
+    IA_HD void finish_moments() {
+        centre();
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) h[3 * i + j] -= cnt * cp[i] * cq[j];
+    }
 };
 
 // TM term 1/(1 + d2/d0sq) (core.h:147,163).

@@ -118,6 +118,10 @@ constexpr long long kOffB2 = kOffW2 + (long long)kHid * kHid;
 constexpr long long kOffW3 = kOffB2 + kHid;
 constexpr long long kOffB3 = kOffW3 + kHid;
 constexpr long long kNParam = kOffB3 + 1;
+// The ranker's output is the network's value plus descriptor column
+// kSkipCol (the twice-refined same-index TM-score): the network learns a
+// correction to that estimate.
+constexpr int kSkipCol = 31;
 
 // The local-fragment work items of one pair (the CI_FRAG enumeration of its
 // chain group) occupy rows [row0, row0 + items) of the feature matrix.
@@ -136,7 +140,7 @@ void launch_frag_features(hipStream_t s, const Arena &A, const FeatGroup *groups
 // C(m,n) = epi(sum_k A(m,k) B(k,n)) with A(m,k) = A[m*sam + k*sak],
 // B(k,n) = B[k*sbk + n*sbn], C(m,n) = C[m*scm + n*scn]; operands rounded to
 // BF16, products accumulated in FP32 on the matrix cores.
-enum GemmEpi { EPI_STORE = 0, EPI_BIAS_RELU = 1, EPI_BIAS = 2, EPI_MASK = 3 };
+enum GemmEpi { EPI_STORE = 0, EPI_BIAS_RELU = 1, EPI_BIAS = 2, EPI_MASK = 3, EPI_BIAS_SKIP = 4 };
 struct GemmArgs {
     const float *A, *B;
     float *C;
@@ -144,6 +148,8 @@ struct GemmArgs {
     int M, N, K, epi;
     const float *bias;          // EPI_BIAS*: bias[n]
     const float *mask;          // EPI_MASK: C *= (mask(m,n) > 0), same strides as C
+    const float *skip;          // EPI_BIAS_SKIP: C += skip[m * sskip] (FP32, not rounded)
+    long long sskip;
 };
 void launch_gemm(hipStream_t s, const GemmArgs &g);
 // dh2(m,n) = dy[m] * w3[n] * (h2(m,n) > 0)

@@ -72,20 +72,28 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         for (int i = 0; i < N; i++) {
             const int jj = i - kk;
             if (jj < 0 || jj >= N || !A.pv[off + i] || !A.nv[off + jj]) continue;
-            ks.add_centroid(A.px[off + i], A.py[off + i], A.pz[off + i], A.qx[off + jj],
-                            A.qy[off + jj], A.qz[off + jj]);
+            if (FAST)
+                ks.add_moment(A.px[off + i], A.py[off + i], A.pz[off + i], A.qx[off + jj],
+                              A.qy[off + jj], A.qz[off + jj]);
+            else
+                ks.add_centroid(A.px[off + i], A.py[off + i], A.pz[off + i], A.qx[off + jj],
+                                A.qy[off + jj], A.qz[off + jj]);
         }
         if (ks.cnt < (double)B.min_ali) {
             out.score = -INFINITY;
             set_identity(out.R, out.t);
             return;
         }
-        ks.centre();
-        for (int i = 0; i < N; i++) {
-            const int jj = i - kk;
-            if (jj < 0 || jj >= N || !A.pv[off + i] || !A.nv[off + jj]) continue;
-            ks.add_cov(A.px[off + i], A.py[off + i], A.pz[off + i], A.qx[off + jj],
-                       A.qy[off + jj], A.qz[off + jj]);
+        if (FAST) {
+            ks.finish_moments();
+        } else {
+            ks.centre();
+            for (int i = 0; i < N; i++) {
+                const int jj = i - kk;
+                if (jj < 0 || jj >= N || !A.pv[off + i] || !A.nv[off + jj]) continue;
+                ks.add_cov(A.px[off + i], A.py[off + i], A.pz[off + i], A.qx[off + jj],
+                           A.qy[off + jj], A.qz[off + jj]);
+            }
         }
         ks.template solve<FAST>(R, t);
     } else {
@@ -97,10 +105,14 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
             if (!in || !pt_valid<PTS>(A, off, i)) continue;
             double x, y, z, qa, qb, qc;
             load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
-            ks.add_centroid(x, y, z, qa, qb, qc);
+            if (FAST) ks.add_moment(x, y, z, qa, qb, qc);
+            else ks.add_centroid(x, y, z, qa, qb, qc);
         }
         if (ks.cnt < 3.0) {
             set_identity(R, t);
+        } else if (FAST) {
+            ks.finish_moments();
+            ks.template solve<FAST>(R, t);
         } else {
             ks.centre();
             for (int i = 0; i < n; i++) {
@@ -120,6 +132,24 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
     for (int it = 0; it < B.iters; it++) {
         ks.reset();
         int cnt = 0;
+        if (FAST) {  // one pass: selection + raw moments
+            for (int i = 0; i < n; i++) {
+                if (!pt_valid<PTS>(A, off, i)) continue;
+                double x, y, z, qa, qb, qc;
+                load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                    cnt++;
+                    ks.add_moment(x, y, z, qa, qb, qc);
+                }
+            }
+            if (cnt >= 3) {
+                ks.finish_moments();
+                ks.template solve<FAST>(R, t);
+            }
+            if (cnt == last) break;
+            last = cnt;
+            continue;
+        }
         for (int i = 0; i < n; i++) {
             if (!pt_valid<PTS>(A, off, i)) continue;
             double x, y, z, qa, qb, qc;

@@ -264,6 +264,44 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
         if (__all(done)) break;
         double c[6] = {0, 0, 0, 0, 0, 0};
         int cnt = 0;
+        if (FAST) {  // one pass: selection + raw second moments
+            double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (!done) {
+                for (int q = s; q < na; q += 8) {
+                    const int ap = L.alP[q], an = L.alN[q];
+                    const double x = A.pcx[off + ap], y = A.pcy[off + ap], z = A.pcz[off + ap];
+                    const double qa = A.qcx[off + an], qb = A.qcy[off + an], qc = A.qcz[off + an];
+                    if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                        cnt++;
+                        c[0] += x; c[1] += y; c[2] += z; c[3] += qa; c[4] += qb; c[5] += qc;
+                        h[0] += x * qa; h[1] += x * qb; h[2] += x * qc;
+                        h[3] += y * qa; h[4] += y * qb; h[5] += y * qc;
+                        h[6] += z * qa; h[7] += z * qb; h[8] += z * qc;
+                    }
+                }
+            }
+            cnt = group8_isum(cnt);
+#pragma unroll
+            for (int d = 0; d < 6; d++) c[d] = group8_sum(c[d]);
+#pragma unroll
+            for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
+            if (!done && cnt >= 3) {
+                const double fc = (double)cnt;
+                const double cp[3] = {c[0] / fc, c[1] / fc, c[2] / fc};
+                const double cq[3] = {c[3] / fc, c[4] / fc, c[5] / fc};
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) h[3 * i + j] -= fc * cp[i] * cq[j];
+                solve_rotation<FAST>(h, R);
+                finish_t(cp, cq, R, t);
+            }
+            if (!done) {
+                if (cnt == last) done = true;
+                last = cnt;
+            }
+            continue;
+        }
         if (!done) {
             for (int q = s; q < na; q += 8) {
                 const int ap = L.alP[q], an = L.alN[q];
@@ -573,8 +611,10 @@ struct LdsPlan {
     }
 };
 
+// At most 256 VGPRs: two wavefronts per SIMD hide the LDS / DPP latency of the
+// fill; measured faster than one wavefront per SIMD with fewer spills.
 template <bool FAST>
-__global__ __launch_bounds__(64) void chain_kernel(Arena A, const ChainGroup *groups,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void chain_kernel(Arena A, const ChainGroup *groups,
                                                    int n_groups, long long n_chains,
                                                    const Hit *xf, Hit *out, ListOut *lout,
                                                    int *lists_p, int *lists_n, int *list_na,

@@ -28,7 +28,10 @@ namespace {
 //     same-index pairing   12 same, register pairing pred a <-> native a+ns-ps
 //  13..23 fraction of same-index residues within the pair's d0 search cutoffs
 //  24 C/11   25 plen/300   26 nlen/300   27 sampled pairs/276   28 log(1+flen)/5
-//  29 register-pairing overlap / ml   30..31 zero
+//  29 register-pairing overlap / ml
+//  30, 31 TM-score (same-index pairing) after one and two refinements of the
+//     superposition: Kabsch over the same-index residues within 2 d0, then
+//     within d0, of the previous superposition
 //  32..95 histogram of |d_pred(i,j) - d_native(i',j')| over up to 24 evenly
 //     spaced fragment positions, 0.25 A bins (last bin open), sum 1.
 constexpr int kHistPos = 24;
@@ -106,6 +109,60 @@ __global__ __launch_bounds__(256) void frag_feature_kernel(Arena A, const FeatGr
         }
         tmr = wave_sum(tmr);
         for (int m = 32; m > 0; m >>= 1) ov += __shfl_xor(ov, m, 64);
+        // two cutoff refinements of the same-index superposition (2 d0, d0)
+        double tmref[2] = {tm, tm};
+        {
+            double Rr[9], tr[3];
+#pragma unroll
+            for (int q = 0; q < 9; q++) Rr[q] = R[q];
+            tr[0] = t[0]; tr[1] = t[1]; tr[2] = t[2];
+            for (int step = 0; step < 2; step++) {
+                const double cut2 = step == 0 ? 4.0 * d0sq : d0sq;
+                double c[6] = {0, 0, 0, 0, 0, 0};
+                int n = 0;
+                for (int i = lane; i < pi.N; i += 64) {
+                    if (!A.v[off + i]) continue;
+                    const double x = A.px[off + i], y = A.py[off + i], z = A.pz[off + i];
+                    const double a = A.qx[off + i], b = A.qy[off + i], cc = A.qz[off + i];
+                    if (dist2_moved(Rr, tr, x, y, z, a, b, cc) >= cut2) continue;
+                    c[0] += x; c[1] += y; c[2] += z; c[3] += a; c[4] += b; c[5] += cc;
+                    n++;
+                }
+                for (int m = 32; m > 0; m >>= 1) n += __shfl_xor(n, m, 64);
+                if (n < 3) {
+                    tmref[step] = step == 0 ? tm : tmref[0];
+                    continue;
+                }
+#pragma unroll
+                for (int d = 0; d < 6; d++) c[d] = wave_sum(c[d]) / (double)n;
+                double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int i = lane; i < pi.N; i += 64) {
+                    if (!A.v[off + i]) continue;
+                    const double x = A.px[off + i], y = A.py[off + i], z = A.pz[off + i];
+                    const double a = A.qx[off + i], b = A.qy[off + i], cc = A.qz[off + i];
+                    if (dist2_moved(Rr, tr, x, y, z, a, b, cc) >= cut2) continue;
+                    const double d0_ = x - c[0], d1_ = y - c[1], d2_ = z - c[2];
+                    const double e0 = a - c[3], e1 = b - c[4], e2 = cc - c[5];
+                    h[0] += d0_ * e0; h[1] += d0_ * e1; h[2] += d0_ * e2;
+                    h[3] += d1_ * e0; h[4] += d1_ * e1; h[5] += d1_ * e2;
+                    h[6] += d2_ * e0; h[7] += d2_ * e1; h[8] += d2_ * e2;
+                }
+#pragma unroll
+                for (int k = 0; k < 9; k++) h[k] = wave_sum(h[k]);
+                solve_rotation<true>(h, Rr);
+                const double cp[3] = {c[0], c[1], c[2]}, cq[3] = {c[3], c[4], c[5]};
+                finish_t(cp, cq, Rr, tr);
+                double sc = 0.0;
+                for (int i = lane; i < pi.N; i += 64) {
+                    if (!A.v[off + i]) continue;
+                    const double d2 = dist2_moved(Rr, tr, A.px[off + i], A.py[off + i],
+                                                  A.pz[off + i], A.qx[off + i], A.qy[off + i],
+                                                  A.qz[off + i]);
+                    sc += 1.0 / (1.0 + d2 / d0sq);
+                }
+                tmref[step] = wave_sum(sc);
+            }
+        }
         // intra-fragment distance-difference histogram
         hist[lane] = 0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -162,6 +219,8 @@ __global__ __launch_bounds__(256) void frag_feature_kernel(Arena A, const FeatGr
                 case 27: f = (float)np / 276.f; break;
                 case 28: f = logf(1.f + (float)flen) / 5.f; break;
                 case 29: f = (float)ov / (float)ml; break;
+                case 30: f = (float)(tmref[0] / L); break;
+                case 31: f = (float)(tmref[1] / L); break;
                 default: break;
             }
 #pragma unroll
@@ -225,6 +284,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         const long long ci = m * g.scm + (long long)n * g.scn;
         if (g.epi == EPI_BIAS_RELU) x = fmaxf(x + g.bias[n], 0.f);
         else if (g.epi == EPI_BIAS) x = x + g.bias[n];
+        else if (g.epi == EPI_BIAS_SKIP) x = x + g.bias[n] + g.skip[m * g.sskip];
         else if (g.epi == EPI_MASK) x = g.mask[ci] > 0.f ? x : 0.f;
         g.C[ci] = x;
     }

@@ -2,7 +2,7 @@
 // GEMM launches (k_learn.hip).  Shapes, with X the n x 96 descriptor rows:
 //   h1 = relu(X W1^T + b1)      n x 256
 //   h2 = relu(h1 W2^T + b2)     n x 256
-//   y  = h2 w3 + b3             n
+//   y  = h2 w3 + b3 + X[:, 31]  n   (residual on the refined same-index TM)
 // Backward of L = mean (y - t)^2:
 //   dy = 2 (y - t) / n,  dh2 = dy w3^T * [h2 > 0],  dh1 = (dh2 W2) * [h1 > 0]
 //   gW2 = dh2^T h1, gW1 = dh1^T X, gw3 = h2^T dy, gb = column sums.
@@ -21,6 +21,7 @@ GemmArgs gemm(const float *A, long long sam, long long sak, const float *B, long
     g.A = A; g.B = B; g.C = C;
     g.sam = sam; g.sak = sak; g.sbk = sbk; g.sbn = sbn; g.scm = scm; g.scn = scn;
     g.M = M; g.N = N; g.K = K; g.epi = epi; g.bias = bias; g.mask = mask;
+    g.skip = nullptr; g.sskip = 0;
     return g;
 }
 
@@ -61,8 +62,11 @@ void Mlp::fwd_chunk(hipStream_t s, const float *X, int n, float *y) {
                         EPI_BIAS_RELU, P_ + kOffB1));
     launch_gemm(s, gemm(h1_, kHid, 1, P_ + kOffW2, 1, kHid, h2_, kHid, 1, n, kHid, kHid,
                         EPI_BIAS_RELU, P_ + kOffB2));
-    launch_gemm(s, gemm(h2_, kHid, 1, P_ + kOffW3, 1, 0, y, 1, 0, n, 1, kHid, EPI_BIAS,
-                        P_ + kOffB3));
+    GemmArgs g3 = gemm(h2_, kHid, 1, P_ + kOffW3, 1, 0, y, 1, 0, n, 1, kHid, EPI_BIAS_SKIP,
+                       P_ + kOffB3);
+    g3.skip = X + kSkipCol;
+    g3.sskip = kFeat;
+    launch_gemm(s, g3);
 }
 
 void Mlp::forward(hipStream_t s, const float *X, long long n, float *y) {

@@ -14,8 +14,9 @@
    decreasing learning rate, MSE on the refined score.
 4. Validation: per pair and pass, whether the best work item is among the
    top-m by predicted score, and how much refined score the top-m miss,
-   next to the same numbers for ranking by the same-index TM-score of the
-   fragment superposition alone (descriptor column 11).
+   next to the same numbers for ranking by single descriptor columns (11:
+   same-index TM-score of the fragment superposition, 12: register pairing,
+   30/31: after one/two cutoff refinements; 31 is the network's skip input).
 
 Writes the parameters (float32, .npy) and a JSON summary beside them.
 """
@@ -115,13 +116,15 @@ def main():
     pred = ia.mlp_forward(params, Xv)
     val_mse = float(np.mean((pred - yv) ** 2))
     dense = Xv[:, 8] > 0.5
-    learned = topm_quality(pred, yv, pv, dense, args.top_m)
-    heuristic = topm_quality(Xv[:, 11], yv, pv, dense, args.top_m)
+    ms = sorted({16, args.top_m})
+    learned = {m: topm_quality(pred, yv, pv, dense, m) for m in ms}
+    heuristic = {f"col{c}": {m: topm_quality(Xv[:, c], yv, pv, dense, m) for m in ms}
+                 for c in (11, 12, 30, 31)}
     summary = {
         "pairs": args.pairs, "first_seed": args.first_seed, "records": int(len(y)),
         "train_rows": int(len(yt)), "val_rows": int(len(yv)), "val_pairs": int(len(val_pairs)),
         "epoch_loss": history, "val_mse": val_mse, "top_m": args.top_m,
-        "val_learned": learned, "val_same_index_tm": heuristic,
+        "val_learned": learned, "val_descriptor_columns": heuristic,
         "seconds": round(time.time() - t0, 1),
     }
     log("[train] " + json.dumps(summary))

@@ -3,9 +3,10 @@
 TEST INFRASTRUCTURE ONLY: tests/ use it as the checker of the GPU kernels
 (indialign_amd/csrc/k_learn.hip, mlp.hip); the product never imports it.
 The ranker has no counterpart in the reference; its definition is the
-library's own (include/indialign.h): 96 -> 256 -> 256 -> 1 with ReLU, MSE
-loss, every matrix-product operand rounded to BF16 (round to nearest even)
-and products accumulated in wide precision.
+library's own (include/indialign.h): 96 -> 256 -> 256 -> 1 with ReLU, plus
+descriptor column 31 added to the output, MSE loss, every matrix-product
+operand rounded to BF16 (round to nearest even) and products accumulated in
+wide precision.
 
 Pins (tests/test_mlp_oracle.py):
   - backward() with rounding off against central finite differences of
@@ -26,6 +27,7 @@ OFF_B2 = OFF_W2 + NHID * NHID
 OFF_W3 = OFF_B2 + NHID
 OFF_B3 = OFF_W3 + NHID
 NPARAM = OFF_B3 + 1
+SKIP_COL = 31  # the output adds this descriptor column (FP32, not rounded)
 
 
 def bf16(a):
@@ -59,7 +61,7 @@ def forward(p, X, rounding=True):
     dt = np.float32 if rounding else np.float64
     h1 = np.maximum(_mm(X, W1.T, rounding) + b1, 0).astype(dt)
     h2 = np.maximum(_mm(h1, W2.T, rounding) + b2, 0).astype(dt)
-    y = (_mm(h2, w3.reshape(-1, 1), rounding)[:, 0] + b3).astype(dt)
+    y = (_mm(h2, w3.reshape(-1, 1), rounding)[:, 0] + b3 + X[:, SKIP_COL]).astype(dt)
     return y, (h1, h2)
 
 

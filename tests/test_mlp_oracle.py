@@ -16,6 +16,8 @@ def test_layout_matches_header():
     hdr = open(os.path.join(ROOT, "include", "indialign.h")).read()
     assert f"#define INDIALIGN_MLP_NPARAM {mo.NPARAM}" in hdr
     assert f"#define INDIALIGN_MLP_NFEAT {mo.NFEAT}" in hdr
+    kh = open(os.path.join(ROOT, "indialign_amd", "csrc", "ia_kernels.h")).read()
+    assert f"constexpr int kSkipCol = {mo.SKIP_COL};" in kh
 
 
 def test_backward_matches_finite_differences():
