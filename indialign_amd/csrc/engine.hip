@@ -191,7 +191,8 @@ class Engine {
 
     // arena
     DBuf b_px, b_py, b_pz, b_qx, b_qy, b_qz, b_v, b_pv, b_nv, b_pcx, b_pcy, b_pcz, b_qcx, b_qcy,
-        b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info, b_qfx, b_qfy, b_qfz;
+        b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info, b_qfx, b_qfy, b_qfz, b_p4, b_q4, b_pc4,
+        b_qc4;
     // work buffers
     DBuf b_seeds, b_masks, b_banks, b_blk_bank, b_blk_first, b_cands, b_bank_out, b_lists_p,
         b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
@@ -202,7 +203,7 @@ class Engine {
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
                 &b_pcz, &b_qcx, &b_qcy, &b_qcz, &b_pmap, &b_nmap, &b_ssp, &b_ssn, &b_info,
-                &b_qfx, &b_qfy, &b_qfz,
+                &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4,
                 &b_seeds, &b_masks, &b_banks, &b_blk_bank, &b_blk_first, &b_cands,
                 &b_bank_out, &b_lists_p, &b_lists_n, &b_list_na, &b_lout, &b_groups, &b_xf,
                 &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
@@ -344,6 +345,19 @@ void Engine::prepare(const NativePairInput *inp, int n) {
     A_.pcx = upload(b_pcx, pcx); A_.pcy = upload(b_pcy, pcy); A_.pcz = upload(b_pcz, pcz);
     A_.qcx = upload(b_qcx, qcx); A_.qcy = upload(b_qcy, qcy); A_.qcz = upload(b_qcz, qcz);
     A_.pmap = upload(b_pmap, pmap); A_.nmap = upload(b_nmap, nmap);
+    {   // 32-byte AoS copies (one wide load per point in the bank kernel)
+        std::vector<P4> p4(px.size()), q4(px.size()), pc4(px.size()), qc4(px.size());
+        for (size_t i = 0; i < px.size(); i++) {
+            p4[i] = P4{px[i], py[i], pz[i], v[i] ? 1.0 : 0.0};
+            q4[i] = P4{qx[i], qy[i], qz[i], 0.0};
+            pc4[i] = P4{pcx[i], pcy[i], pcz[i], 0.0};
+            qc4[i] = P4{qcx[i], qcy[i], qcz[i], 0.0};
+        }
+        A_.p4 = upload(b_p4, p4);
+        A_.q4 = upload(b_q4, q4);
+        A_.pc4 = upload(b_pc4, pc4);
+        A_.qc4 = upload(b_qc4, qc4);
+    }
     {   // FP32 copies of the compact native coordinates (fast method)
         std::vector<float> fx(qcx.begin(), qcx.end()), fy(qcy.begin(), qcy.end()),
             fz(qcz.begin(), qcz.end());

@@ -37,6 +37,12 @@ struct PairInfo {
 
 // Coordinate arena: SoA FP64 for pred (p*) and native (q*), masks, compact
 // pred/native coordinates (pc*, qc*) and compact->full index maps.
+// One point, 32 bytes: a single wide (scalar or vector) load.  w is the
+// valid flag (1.0 / 0.0) in the full-index pred copy, 0 elsewhere.
+struct alignas(32) P4 {
+    double x, y, z, w;
+};
+
 struct Arena {
     const double *px, *py, *pz, *qx, *qy, *qz;
     const uint8_t *v, *pv, *nv;
@@ -46,6 +52,8 @@ struct Arena {
     const char *ssp, *ssn;                             // secondary structure (full index)
     const PairInfo *info;
     const int *isel;                                   // selected work items (learned method)
+    const P4 *p4, *q4;                                 // full-index AoS copies (p4.w = valid)
+    const P4 *pc4, *qc4;                               // compact AoS copies
 };
 
 IA_HD void set_identity(double *R, double *t) {
@@ -69,6 +77,16 @@ IA_HD double dist2_moved(const double *R, const double *t, double x, double y, d
     xform(R, t, x, y, z, mx, my, mz);
     const double dx = mx - qx, dy = my - qy, dz = mz - qz;
     return dx * dx + dy * dy + dz * dz;
+}
+
+// Same distance with fused multiply-adds (fast method only: not the
+// reference's rounding).
+IA_HD double dist2_moved_fma(const double *R, const double *t, double x, double y, double z,
+                             double qx, double qy, double qz) {
+    const double dx = fma(x, R[0], fma(y, R[3], fma(z, R[6], t[0]))) - qx;
+    const double dy = fma(x, R[1], fma(y, R[4], fma(z, R[7], t[1]))) - qy;
+    const double dz = fma(x, R[2], fma(y, R[5], fma(z, R[8], t[2]))) - qz;
+    return fma(dx, dx, fma(dy, dy, dz * dz));
 }
 
 // Cyclic Jacobi, 8 sweeps, pair order (0,1)(0,2)(0,3)(1,2)(1,3)(2,3),
@@ -298,9 +316,9 @@ struct KabschSeq {
         cp[0] += x; cp[1] += y; cp[2] += z;
         cq[0] += a; cq[1] += b; cq[2] += c;
         cnt += 1.0;
-        h[0] += x * a; h[1] += x * b; h[2] += x * c;
-        h[3] += y * a; h[4] += y * b; h[5] += y * c;
-        h[6] += z * a; h[7] += z * b; h[8] += z * c;
+        h[0] = fma(x, a, h[0]); h[1] = fma(x, b, h[1]); h[2] = fma(x, c, h[2]);
+        h[3] = fma(y, a, h[3]); h[4] = fma(y, b, h[4]); h[5] = fma(y, c, h[5]);
+        h[6] = fma(z, a, h[6]); h[7] = fma(z, b, h[7]); h[8] = fma(z, c, h[8]);
     }
     IA_HD void finish_moments() {
         centre();

@@ -14,26 +14,26 @@ namespace ia {
 
 namespace {
 
+// Point i of the bank's point set: one 32-byte load per side (wave-uniform
+// address, so a scalar load).  Returns false for an invalid residue
+// (PTS_FULL only).
 template <int PTS>
-IA_D void load_pt(const Arena &A, int off, const int *lp, const int *ln, int a0, int a1, int i,
-                  double &x, double &y, double &z, double &qa, double &qb, double &qc) {
+IA_D bool get_pt(const Arena &A, int off, const int *lp, const int *ln, int a0, int a1, int i,
+                 double &x, double &y, double &z, double &qa, double &qb, double &qc) {
+    P4 p, q;
     if (PTS == PTS_FULL) {
-        x = A.px[off + i]; y = A.py[off + i]; z = A.pz[off + i];
-        qa = A.qx[off + i]; qb = A.qy[off + i]; qc = A.qz[off + i];
+        p = A.p4[off + i];
+        q = A.q4[off + i];
     } else if (PTS == PTS_LIST) {
-        const int ip = off + lp[i], in = off + ln[i];
-        x = A.px[ip]; y = A.py[ip]; z = A.pz[ip];
-        qa = A.qx[in]; qb = A.qy[in]; qc = A.qz[in];
+        p = A.p4[off + lp[i]];
+        q = A.q4[off + ln[i]];
     } else {
-        const int ip = off + a0 + i, in = off + a1 + i;
-        x = A.pcx[ip]; y = A.pcy[ip]; z = A.pcz[ip];
-        qa = A.qcx[in]; qb = A.qcy[in]; qc = A.qcz[in];
+        p = A.pc4[off + a0 + i];
+        q = A.qc4[off + a1 + i];
     }
-}
-
-template <int PTS>
-IA_D bool pt_valid(const Arena &A, int off, int i) {
-    return PTS == PTS_FULL ? A.v[off + i] != 0 : true;
+    x = p.x; y = p.y; z = p.z;
+    qa = q.x; qb = q.y; qc = q.z;
+    return PTS == PTS_FULL ? p.w != 0.0 : true;
 }
 
 IA_D int bank_points(const BankDesc &B, const int *list_na) {
@@ -102,9 +102,9 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         const uint8_t *m = masks + (win ? 0 : (long long)sd.a);
         for (int i = 0; i < n; i++) {
             const bool in = win ? (i >= sd.a && i < sd.a + sd.b) : (m[i] != 0);
-            if (!in || !pt_valid<PTS>(A, off, i)) continue;
+            if (!in) continue;
             double x, y, z, qa, qb, qc;
-            load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+            if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
             if (FAST) ks.add_moment(x, y, z, qa, qb, qc);
             else ks.add_centroid(x, y, z, qa, qb, qc);
         }
@@ -117,9 +117,9 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
             ks.centre();
             for (int i = 0; i < n; i++) {
                 const bool in = win ? (i >= sd.a && i < sd.a + sd.b) : (m[i] != 0);
-                if (!in || !pt_valid<PTS>(A, off, i)) continue;
+                if (!in) continue;
                 double x, y, z, qa, qb, qc;
-                load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+                if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
                 ks.add_cov(x, y, z, qa, qb, qc);
             }
             ks.template solve<FAST>(R, t);
@@ -134,10 +134,9 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         int cnt = 0;
         if (FAST) {  // one pass: selection + raw moments
             for (int i = 0; i < n; i++) {
-                if (!pt_valid<PTS>(A, off, i)) continue;
                 double x, y, z, qa, qb, qc;
-                load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
-                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+                if (dist2_moved_fma(R, t, x, y, z, qa, qb, qc) <= lim) {
                     cnt++;
                     ks.add_moment(x, y, z, qa, qb, qc);
                 }
@@ -151,9 +150,8 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
             continue;
         }
         for (int i = 0; i < n; i++) {
-            if (!pt_valid<PTS>(A, off, i)) continue;
             double x, y, z, qa, qb, qc;
-            load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+            if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
             if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
                 cnt++;
                 ks.add_centroid(x, y, z, qa, qb, qc);
@@ -162,9 +160,8 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         if (cnt >= 3) {
             ks.centre();
             for (int i = 0; i < n; i++) {
-                if (!pt_valid<PTS>(A, off, i)) continue;
                 double x, y, z, qa, qb, qc;
-                load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+                if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
                 if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) ks.add_cov(x, y, z, qa, qb, qc);
             }
             ks.template solve<FAST>(R, t);
@@ -176,10 +173,10 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
     const double d0sq = fmax(pi.d0 * pi.d0, 1e-12), c8 = B.sd8 * B.sd8;
     double acc = 0.0;
     for (int i = 0; i < n; i++) {
-        if (!pt_valid<PTS>(A, off, i)) continue;
         double x, y, z, qa, qb, qc;
-        load_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc);
-        const double d2 = dist2_moved(R, t, x, y, z, qa, qb, qc);
+        if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+        const double d2 = FAST ? dist2_moved_fma(R, t, x, y, z, qa, qb, qc)
+                               : dist2_moved(R, t, x, y, z, qa, qb, qc);
         if (B.use_d8 && !(d2 <= c8)) continue;
         acc += tm_term(d2, d0sq);
     }

@@ -62,6 +62,7 @@ IA_D double ror1(double v) {
     const int lo = ror1(__double2loint(v)), hi = ror1(__double2hiint(v));
     return __hiloint2double(hi, lo);
 }
+IA_D float ror1(float v) { return __int_as_float(ror1(__float_as_int(v))); }
 
 // Needleman-Wunsch fill (nw.h:10-38) over the compact plen x nlen matrix,
 // score computed on the fly.
@@ -134,6 +135,23 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
                 XH[j] = ror1(H[j]);
                 XG[j] = ror1(G[j]);
             }
+            // FP32: the next step's columns flow down the lanes with the
+            // cells (lane l's next column is lane l-1's current one); only
+            // slot 0's lane 0 reads a new column, at a wave-uniform address.
+            V Xx[K], Xy[K], Xz[K], c0x = 0, c0y = 0, c0z = 0;
+            if constexpr (F32) {
+                if (SMODE != SM_SS) {
+#pragma unroll
+                    for (int j = 0; j < K; j++) {
+                        Xx[j] = ror1(cx[j]);
+                        Xy[j] = ror1(cy[j]);
+                        Xz[j] = ror1(cz[j]);
+                    }
+                    c0x = colx(rn);
+                    c0y = coly(rn);
+                    c0z = colz(rn);
+                }
+            }
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 const int b = st - 64 * j - lane;
@@ -142,17 +160,29 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
                 const int bn = min(max(b + 1, 0), nlen - 1);
                 V nx = 0, ny = 0, nz = 0;
                 char ns = 0;
-                if (SMODE != SM_SS) { nx = colx(bn); ny = coly(bn); nz = colz(bn); }
+                if (SMODE != SM_SS) {
+                    if constexpr (F32) {
+                        nx = lane == 0 ? (j == 0 ? c0x : Xx[j > 0 ? j - 1 : 0]) : Xx[j];
+                        ny = lane == 0 ? (j == 0 ? c0y : Xy[j > 0 ? j - 1 : 0]) : Xy[j];
+                        nz = lane == 0 ? (j == 0 ? c0z : Xz[j > 0 ? j - 1 : 0]) : Xz[j];
+                    } else {
+                        nx = colx(bn); ny = coly(bn); nz = colz(bn);
+                    }
+                }
                 if (SMODE != SM_TM) ns = cols(bn);
                 V s;
                 if (SMODE == SM_SS) {
                     s = (sp[j] == cs[j]) ? (V)1 : (V)0;
                 } else {
                     const V dx = mx[j] - cx[j], dy = my[j] - cy[j], dz = mz[j] - cz[j];
-                    const V d2 = dx * dx + dy * dy + dz * dz;
-                    V tm;
-                    if constexpr (F32) tm = __builtin_amdgcn_rcpf(1.0f + d2 * inv_d0sq);
-                    else tm = 1.0 / (1.0 + d2 / d0sq_s);
+                    V d2, tm;
+                    if constexpr (F32) {
+                        d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                        tm = __builtin_amdgcn_rcpf(__builtin_fmaf(d2, inv_d0sq, 1.0f));
+                    } else {
+                        d2 = dx * dx + dy * dy + dz * dz;
+                        tm = 1.0 / (1.0 + d2 / d0sq_s);
+                    }
                     if (SMODE == SM_TM) {
                         s = (d2 <= cut8) ? tm : (V)0;
                     } else {
@@ -271,12 +301,12 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
                     const int ap = L.alP[q], an = L.alN[q];
                     const double x = A.pcx[off + ap], y = A.pcy[off + ap], z = A.pcz[off + ap];
                     const double qa = A.qcx[off + an], qb = A.qcy[off + an], qc = A.qcz[off + an];
-                    if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                    if (dist2_moved_fma(R, t, x, y, z, qa, qb, qc) <= lim) {
                         cnt++;
                         c[0] += x; c[1] += y; c[2] += z; c[3] += qa; c[4] += qb; c[5] += qc;
-                        h[0] += x * qa; h[1] += x * qb; h[2] += x * qc;
-                        h[3] += y * qa; h[4] += y * qb; h[5] += y * qc;
-                        h[6] += z * qa; h[7] += z * qb; h[8] += z * qc;
+                        h[0] = fma(x, qa, h[0]); h[1] = fma(x, qb, h[1]); h[2] = fma(x, qc, h[2]);
+                        h[3] = fma(y, qa, h[3]); h[4] = fma(y, qb, h[4]); h[5] = fma(y, qc, h[5]);
+                        h[6] = fma(z, qa, h[6]); h[7] = fma(z, qb, h[7]); h[8] = fma(z, qc, h[8]);
                     }
                 }
             }
