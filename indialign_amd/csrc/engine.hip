@@ -158,6 +158,7 @@ class Engine {
     std::vector<signed char> status_;  // per loaded pair: 0 search, 1 Lnorm<=2, -1 N<=0
     Arena A_{};
     int list_cap_ = 1;
+    int n_res_ = 0;  // residues of all loaded pairs
 
     void ensure_lists(int slots, int cap) {
         list_cap_ = std::max(cap, 1);
@@ -192,7 +193,7 @@ class Engine {
     // arena
     DBuf b_px, b_py, b_pz, b_qx, b_qy, b_qz, b_v, b_pv, b_nv, b_pcx, b_pcy, b_pcz, b_qcx, b_qcy,
         b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info, b_qfx, b_qfy, b_qfz, b_p4, b_q4, b_pc4,
-        b_qc4;
+        b_qc4, b_ss_scr;
     // work buffers
     DBuf b_seeds, b_masks, b_banks, b_blk_bank, b_blk_first, b_cands, b_bank_out, b_lists_p,
         b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
@@ -203,7 +204,7 @@ class Engine {
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
                 &b_pcz, &b_qcx, &b_qcy, &b_qcz, &b_pmap, &b_nmap, &b_ssp, &b_ssn, &b_info,
-                &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4,
+                &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4, &b_ss_scr,
                 &b_seeds, &b_masks, &b_banks, &b_blk_bank, &b_blk_first, &b_cands,
                 &b_bank_out, &b_lists_p, &b_lists_n, &b_list_na, &b_lout, &b_groups, &b_xf,
                 &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
@@ -366,6 +367,7 @@ void Engine::prepare(const NativePairInput *inp, int n) {
         A_.qfz = upload(b_qfz, fz);
         IA_CHECK(hipStreamSynchronize(s_));
     }
+    n_res_ = off;
     A_.ssp = b_ssp.get<char>(std::max(off, 1));
     A_.ssn = b_ssn.get<char>(std::max(off, 1));
     A_.info = upload(b_info, info);
@@ -724,7 +726,8 @@ void Engine::stage_ss() {
             lmax = std::max(lmax, std::min(hp_[a].plen, hp_[a].nlen));
         }
     if (!E.empty()) {
-        launch_ss(s_, A_, (int)hp_.size(), (char *)A_.ssp, (char *)A_.ssn);
+        launch_ss(s_, A_, (int)hp_.size(), (char *)A_.ssp, (char *)A_.ssn,
+                  b_ss_scr.get<int>((size_t)2 * 9 * std::max(n_res_, 1)));
         launched();
         IA_CHECK(hipGetLastError());
         const std::vector<int> kmax = max_seed_table(lmax, 50);

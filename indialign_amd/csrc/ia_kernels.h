@@ -96,7 +96,8 @@ void launch_topk(hipStream_t s, const Arena &A, const int *pairs, const Hit *bes
                  const long long *mask_off, uint8_t *masks);
 void launch_weighted(hipStream_t s, const Arena &A, const int *pairs, const Hit *best, int n,
                      Hit *out);
-void launch_ss(hipStream_t s, const Arena &A, int n_pairs, char *ssp, char *ssn);
+// scratch: 2 * (sum of N) * 9 ints
+void launch_ss(hipStream_t s, const Arena &A, int n_pairs, char *ssp, char *ssn, int *scratch);
 void launch_anchors(hipStream_t s, const Arena &A, const int *pairs, int n,
                     const SeedDesc *frag_seeds, const int *frag_off, const int *frag_cnt,
                     uint8_t *anchor_masks, const long long *anchor_off, int *anchor_cnt,

@@ -178,7 +178,8 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         const double d2 = FAST ? dist2_moved_fma(R, t, x, y, z, qa, qb, qc)
                                : dist2_moved(R, t, x, y, z, qa, qb, qc);
         if (B.use_d8 && !(d2 <= c8)) continue;
-        acc += tm_term(d2, d0sq);
+        // fast: one divide, d0^2 / (d0^2 + d2), instead of the reference's two
+        acc += FAST ? d0sq / (d0sq + d2) : tm_term(d2, d0sq);
     }
 #pragma unroll
     for (int q = 0; q < 9; q++) out.R[q] = R[q];

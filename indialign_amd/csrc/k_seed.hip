@@ -158,55 +158,97 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
 // largest j > i+3 such that (i,j) and (i+1,j-1) are 9-12 A apart and both
 // are still unpaired.  Side 0 = pred with pred-valid, 1 = native with
 // native-valid.  Writes 'L', '<', '>' by full index.
-__global__ __launch_bounds__(256) void ss_kernel(Arena A, char *ssp, char *ssn) {
+//
+// The geometric test does not depend on the pairing state, so the block
+// first finds, in parallel, the kSsCand largest qualifying j of every i
+// (cand, count; a count above kSsCand means more exist); one thread then runs
+// the greedy pass over those lists in LDS, rescanning only when all stored
+// candidates of an i are already paired.
+constexpr int kSsCand = 8;
+constexpr int kSsLds = 8192;
+
+IA_D bool stem_geom(const double *X, const double *Y, const double *Z, int off, int i, int j) {
+    const double dx = X[off + i] - X[off + j], dy = Y[off + i] - Y[off + j],
+                 dz = Z[off + i] - Z[off + j];
+    const double a = dx * dx + dy * dy + dz * dz;
+    const double ex = X[off + i + 1] - X[off + j - 1], ey = Y[off + i + 1] - Y[off + j - 1],
+                 ez = Z[off + i + 1] - Z[off + j - 1];
+    const double b = ex * ex + ey * ey + ez * ez;
+    return a >= 81.0 && a <= 144.0 && b >= 81.0 && b <= 144.0;
+}
+
+__global__ __launch_bounds__(256) void ss_kernel(Arena A, char *ssp, char *ssn, int *scratch) {
     const int p = blockIdx.x >> 1, side = blockIdx.x & 1;
     const PairInfo &pi = A.info[p];
     const int N = pi.N, off = pi.off, tid = threadIdx.x;
     const double *X = side ? A.qx : A.px;
     const double *Y = side ? A.qy : A.py;
     const double *Z = side ? A.qz : A.pz;
-    const uint8_t *v = side ? A.nv : A.pv;
-    char *ss = (side ? ssn : ssp) + off;
-    __shared__ int best;
-    // mates are encoded in ss[] as: 'L' = free, '<' / '>' = paired
-    for (int i = tid; i < N; i += 256) ss[i] = 'L';
-    __threadfence();
-    __syncthreads();
-    for (int i = 0; i + 4 < N; i++) {
-        if (!v[off + i] || !v[off + i + 1]) continue;
-        if (ss[i] != 'L') continue;  // i already paired: no j can pass
-        int bj = -1;
-        const double xi = X[off + i], yi = Y[off + i], zi = Z[off + i];
-        const double xi1 = X[off + i + 1], yi1 = Y[off + i + 1], zi1 = Z[off + i + 1];
-        for (int j = N - 1 - tid; j > i + 3; j -= 256) {
-            if (!v[off + j] || !v[off + j - 1] || ss[j] != 'L') continue;
-            const double dx = xi - X[off + j], dy = yi - Y[off + j], dz = zi - Z[off + j];
-            const double a = dx * dx + dy * dy + dz * dz;
-            const double ex = xi1 - X[off + j - 1], ey = yi1 - Y[off + j - 1], ez = zi1 - Z[off + j - 1];
-            const double b = ex * ex + ey * ey + ez * ez;
-            if (a >= 81.0 && a <= 144.0 && b >= 81.0 && b <= 144.0) {
-                bj = j;
-                break;
+    const uint8_t *vg = (side ? A.nv : A.pv) + off;
+    char *ssg = (side ? ssn : ssp) + off;
+    int *cand = scratch + ((long long)2 * off + (long long)side * N) * (kSsCand + 1);
+    int *count = cand + (long long)N * kSsCand;
+    __shared__ char ss_l[kSsLds];
+    __shared__ uint8_t v_l[kSsLds];
+    const bool lds = N <= kSsLds;
+    char *ss = lds ? ss_l : ssg;
+    const uint8_t *v = lds ? v_l : vg;
+    for (int i = tid; i < N; i += 256) {
+        ss[i] = 'L';
+        if (lds) v_l[i] = vg[i];
+    }
+    // phase 1: candidate lists
+    for (int i = tid; i + 4 < N; i += 256) {
+        int k = 0;
+        if (vg[i] && vg[i + 1]) {
+            for (int j = N - 1; j > i + 3 && k <= kSsCand; j--) {
+                if (!vg[j] || !vg[j - 1]) continue;
+                if (!stem_geom(X, Y, Z, off, i, j)) continue;
+                if (k < kSsCand) cand[(long long)i * kSsCand + k] = j;
+                k++;
             }
         }
-        if (tid == 0) best = -1;
-        __syncthreads();
-        if (bj >= 0) atomicMax(&best, bj);
-        __syncthreads();
-        if (tid == 0 && best >= 0) {
-            ss[i] = '<';
-            ss[best] = '>';
-        }
-        __threadfence();
-        __syncthreads();
+        count[i] = k;
     }
+    __threadfence_block();
+    __syncthreads();
+    // phase 2: the greedy pass, in order
+    if (tid == 0) {
+        for (int i = 0; i + 4 < N; i++) {
+            if (!v[i] || !v[i + 1] || ss[i] != 'L') continue;
+            const int k = count[i];
+            const int *c = cand + (long long)i * kSsCand;
+            int best = -1;
+            for (int q = 0; q < k && q < kSsCand; q++)
+                if (ss[c[q]] == 'L') {
+                    best = c[q];
+                    break;
+                }
+            if (best < 0 && k > kSsCand) {
+                for (int j = c[kSsCand - 1] - 1; j > i + 3; j--) {
+                    if (!v[j] || !v[j - 1] || ss[j] != 'L') continue;
+                    if (stem_geom(X, Y, Z, off, i, j)) {
+                        best = j;
+                        break;
+                    }
+                }
+            }
+            if (best >= 0) {
+                ss[i] = '<';
+                ss[best] = '>';
+            }
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (int i = tid; i < N; i += 256) ssg[i] = ss_l[i];
 }
 
 }  // namespace
 
-void launch_ss(hipStream_t s, const Arena &A, int n_pairs, char *ssp, char *ssn) {
+void launch_ss(hipStream_t s, const Arena &A, int n_pairs, char *ssp, char *ssn, int *scratch) {
     if (n_pairs <= 0) return;
-    ss_kernel<<<2 * n_pairs, 256, 0, s>>>(A, ssp, ssn);
+    ss_kernel<<<2 * n_pairs, 256, 0, s>>>(A, ssp, ssn, scratch);
 }
 
 void launch_anchors(hipStream_t s, const Arena &A, const int *pairs, int n,
