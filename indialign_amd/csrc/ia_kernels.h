@@ -153,6 +153,11 @@ struct GemmArgs {
     long long sskip;
 };
 void launch_gemm(hipStream_t s, const GemmArgs &g);
+// Ranker inference fused in one kernel (same rounding points as the GEMM
+// path): y[0..n) from X rows, parameters P and their BF16 weight copies.
+void launch_weights_bf16(hipStream_t s, const float *P, short *W1b, short *W2b);
+void launch_mlp_infer(hipStream_t s, const float *X, long long n, const short *W1b,
+                      const short *W2b, const float *P, float *y);
 // dh2(m,n) = dy[m] * w3[n] * (h2(m,n) > 0)
 void launch_head_back(hipStream_t s, const float *dy, const float *w3, const float *h2, int n,
                       float *dh2);

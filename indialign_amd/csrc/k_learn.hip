@@ -242,6 +242,8 @@ IA_D short bf16_rne(float f) {
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (short)(u >> 16);
 }
+// The FP32 value of bf16_rne(f).
+IA_D float bf16_val(float f) { return __uint_as_float((unsigned)(unsigned short)bf16_rne(f) << 16); }
 
 // One workgroup = 4 wavefronts = a 64 x 64 tile of C; each wavefront owns a
 // 32 x 32 sub-tile and runs v_mfma_f32_32x32x8_bf16 over K in steps of 8.
@@ -288,6 +290,104 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         else if (g.epi == EPI_MASK) x = g.mask[ci] > 0.f ? x : 0.f;
         g.C[ci] = x;
     }
+}
+
+// ---- fused ranker inference ------------------------------------------------
+// One wavefront = 32 rows, a workgroup = 4 wavefronts.  Layer 1
+// (32x96 . 96x256) accumulates in 8 tiles of 32x32; relu(+b1) goes to LDS as
+// BF16; layer 2 (32x256 . 256x256) reads its A operand from there; the head
+// relu(+b2) . w3 is reduced across the 32 lanes of each column group.  The
+// rounding points are those of the GEMM path: every matrix operand
+// (X, W1, h1, W2, h2, w3) in BF16, sums in FP32.
+constexpr int kInferRows = 128;
+constexpr int kH1Stride = 260;  // BF16 elements per LDS row (8-byte aligned, 2-way banks)
+
+__global__ __launch_bounds__(256) void mlp_infer_kernel(const float *X, int n, const short *W1b,
+                                                        const short *W2b, const float *P,
+                                                        float *y) {
+    __shared__ short h1s[4][32 * kH1Stride];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 31, kq = (lane >> 5) * 4;
+    const long long m0 = (long long)blockIdx.x * kInferRows + wv * 32;
+    const long long row = m0 + r;
+    const bool rok = row < n;
+    short *H = h1s[wv];
+    f32x16 acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[t][v] = 0.f;
+    // layer 1
+    for (int k0 = 0; k0 < kFeat; k0 += 8) {
+        bf16x4 a;
+        if (rok) {
+            const float4 xa = *(const float4 *)(X + row * kFeat + k0 + kq);
+            a[0] = bf16_rne(xa.x); a[1] = bf16_rne(xa.y); a[2] = bf16_rne(xa.z); a[3] = bf16_rne(xa.w);
+        } else {
+            a[0] = a[1] = a[2] = a[3] = 0;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const bf16x4 b = *(const bf16x4 *)(W1b + (t * 32 + r) * kFeat + k0 + kq);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+    }
+    // relu(+b1) -> LDS (BF16); result v of lane l is row 8(v/4)+4(l/32)+v%4, col l%32
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int col = t * 32 + r;
+        const float bias = P[kOffB1 + col];
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int rr = 8 * (v >> 2) + (lane >> 5) * 4 + (v & 3);
+            H[rr * kH1Stride + col] = bf16_rne(fmaxf(acc[t][v] + bias, 0.f));
+            acc[t][v] = 0.f;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // layer 2
+    for (int k0 = 0; k0 < kHid; k0 += 8) {
+        const bf16x4 a = *(const bf16x4 *)(H + r * kH1Stride + k0 + kq);
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const bf16x4 b = *(const bf16x4 *)(W2b + (t * 32 + r) * kHid + k0 + kq);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+    }
+    // head: sum over columns of bf16(relu(h2 + b2)) * bf16(w3), per row
+    float part[16];
+#pragma unroll
+    for (int v = 0; v < 16; v++) part[v] = 0.f;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        const int col = t * 32 + r;
+        const float bias = P[kOffB2 + col];
+        const float w = bf16_val(P[kOffW3 + col]);
+#pragma unroll
+        for (int v = 0; v < 16; v++) part[v] += bf16_val(fmaxf(acc[t][v] + bias, 0.f)) * w;
+    }
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        float x = part[v];
+        for (int o = 16; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        part[v] = x;
+    }
+    if (r == 0) {
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const long long rr = m0 + 8 * (v >> 2) + (lane >> 5) * 4 + (v & 3);
+            if (rr < n) y[rr] = part[v] + P[kOffB3] + X[rr * kFeat + kSkipCol];
+        }
+    }
+}
+
+// W1, W2 of the parameter array -> BF16 copies.
+__global__ void weights_bf16_kernel(const float *P, short *W1b, short *W2b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kHid * kFeat) W1b[i] = bf16_rne(P[kOffW1 + i]);
+    if (i < kHid * kHid) W2b[i] = bf16_rne(P[kOffW2 + i]);
 }
 
 __global__ void head_back_kernel(const float *dy, const float *w3, const float *h2, long long n,
@@ -461,6 +561,20 @@ void launch_gemm(hipStream_t s, const GemmArgs &g) {
     if (g.M <= 0 || g.N <= 0) return;
     const dim3 grid((unsigned)((g.N + 63) / 64), (unsigned)((g.M + 63) / 64));
     hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, s, g);
+}
+
+void launch_weights_bf16(hipStream_t s, const float *P, short *W1b, short *W2b) {
+    hipLaunchKernelGGL(weights_bf16_kernel, dim3((kHid * kHid + 255) / 256), dim3(256), 0, s, P,
+                       W1b, W2b);
+}
+
+void launch_mlp_infer(hipStream_t s, const float *X, long long n, const short *W1b,
+                      const short *W2b, const float *P, float *y) {
+    for (long long r0 = 0; r0 < n; r0 += (1LL << 30)) {
+        const int m = (int)((n - r0) < (1LL << 30) ? (n - r0) : (1LL << 30));
+        hipLaunchKernelGGL(mlp_infer_kernel, dim3((m + kInferRows - 1) / kInferRows), dim3(256), 0,
+                           s, X + r0 * kFeat, m, W1b, W2b, P, y + r0);
+    }
 }
 
 void launch_head_back(hipStream_t s, const float *dy, const float *w3, const float *h2, int n,

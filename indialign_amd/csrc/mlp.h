@@ -18,8 +18,11 @@ public:
     void get_params(hipStream_t s, float *host);
     bool loaded() const { return loaded_; }
     void unload() { loaded_ = false; }
-    // y[0..n) = f(X[0..n) rows); X and y on the device; any n (chunked).
+    // y[0..n) = f(X[0..n) rows); X and y on the device; any n.  One fused
+    // matrix-core kernel (inference).
     void forward(hipStream_t s, const float *X, long long n, float *y);
+    // The same through the training GEMM sequence (chunked).
+    void forward_gemm(hipStream_t s, const float *X, long long n, float *y);
     // Gradient of mean (f(X) - t)^2 over n <= kChunk rows into grad(), and
     // the loss into loss_dev()[0].
     void backward(hipStream_t s, const float *X, const float *t, int n);
@@ -34,7 +37,9 @@ public:
 private:
     void alloc();
     void fwd_chunk(hipStream_t s, const float *X, int n, float *y);
-    DBuf bP_, bG_, bM_, bV_, bh1_, bh2_, bdh1_, bdh2_, bones_, by_, bdy_, bloss_;
+    DBuf bP_, bG_, bM_, bV_, bh1_, bh2_, bdh1_, bdh2_, bones_, by_, bdy_, bloss_, bW1b_, bW2b_;
+    short *W1b_ = nullptr, *W2b_ = nullptr;
+    bool wb_dirty_ = true;
     float *P_ = nullptr, *G_ = nullptr, *M_ = nullptr, *V_ = nullptr;
     float *h1_ = nullptr, *h2_ = nullptr, *dh1_ = nullptr, *dh2_ = nullptr;
     float *ones_ = nullptr, *y_ = nullptr, *dy_ = nullptr, *loss_ = nullptr;

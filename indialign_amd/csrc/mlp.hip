@@ -41,6 +41,8 @@ void Mlp::alloc() {
     y_ = by_.get<float>(kChunk);
     dy_ = bdy_.get<float>(kChunk);
     loss_ = bloss_.get<float>(4);
+    W1b_ = bW1b_.get<short>((size_t)kHid * kFeat);
+    W2b_ = bW2b_.get<short>((size_t)kHid * kHid);
     std::vector<float> one(kChunk, 1.0f);
     IA_CHECK(hipMemcpy(ones_, one.data(), sizeof(float) * kChunk, hipMemcpyHostToDevice));
 }
@@ -50,6 +52,7 @@ void Mlp::set_params(hipStream_t s, const float *host) {
     IA_CHECK(hipMemcpyAsync(P_, host, sizeof(float) * kNParam, hipMemcpyHostToDevice, s));
     IA_CHECK(hipStreamSynchronize(s));
     loaded_ = true;
+    wb_dirty_ = true;
 }
 
 void Mlp::get_params(hipStream_t s, float *host) {
@@ -70,6 +73,17 @@ void Mlp::fwd_chunk(hipStream_t s, const float *X, int n, float *y) {
 }
 
 void Mlp::forward(hipStream_t s, const float *X, long long n, float *y) {
+    alloc();
+    if (n <= 0) return;
+    if (wb_dirty_) {
+        launch_weights_bf16(s, P_, W1b_, W2b_);
+        wb_dirty_ = false;
+    }
+    launch_mlp_infer(s, X, n, W1b_, W2b_, P_, y);
+    IA_CHECK(hipGetLastError());
+}
+
+void Mlp::forward_gemm(hipStream_t s, const float *X, long long n, float *y) {
     alloc();
     for (long long r = 0; r < n; r += kChunk) {
         const int m = (int)std::min<long long>(kChunk, n - r);
@@ -107,6 +121,7 @@ void Mlp::reset_adam(hipStream_t s) {
 
 void Mlp::adam_step(hipStream_t s, float lr) {
     step_++;
+    wb_dirty_ = true;
     launch_adam(s, P_, G_, M_, V_, kNParam, lr, 0.9f, 0.999f, 1e-8f, step_);
     IA_CHECK(hipGetLastError());
 }
