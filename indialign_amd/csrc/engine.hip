@@ -193,7 +193,7 @@ class Engine {
     // arena
     DBuf b_px, b_py, b_pz, b_qx, b_qy, b_qz, b_v, b_pv, b_nv, b_pcx, b_pcy, b_pcz, b_qcx, b_qcy,
         b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info, b_qfx, b_qfy, b_qfz, b_p4, b_q4, b_pc4,
-        b_qc4, b_ss_scr;
+        b_qc4, b_ss_scr, b_fblk_g, b_fblk_f;
     // work buffers
     DBuf b_seeds, b_masks, b_banks, b_blk_bank, b_blk_first, b_cands, b_bank_out, b_lists_p,
         b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
@@ -204,7 +204,7 @@ class Engine {
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
                 &b_pcz, &b_qcx, &b_qcy, &b_qcz, &b_pmap, &b_nmap, &b_ssp, &b_ssn, &b_info,
-                &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4, &b_ss_scr,
+                &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4, &b_ss_scr, &b_fblk_g, &b_fblk_f,
                 &b_seeds, &b_masks, &b_banks, &b_blk_bank, &b_blk_first, &b_cands,
                 &b_bank_out, &b_lists_p, &b_lists_n, &b_list_na, &b_lout, &b_groups, &b_xf,
                 &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
@@ -899,7 +899,14 @@ void Engine::stage_rescue() {
         if ((rank || rec) && rows > 0) {
             X = b_feat.get<float>((size_t)rows * kFeat);
             const FeatGroup *fdev = upload(b_fgroups, fgs);
-            launch_frag_features(s_, A_, fdev, (int)fgs.size(), rows, X);
+            std::vector<int> bg, bf;
+            for (size_t k = 0; k < fgs.size(); k++)
+                for (int f = 0; f < fgs[k].items; f += kFeatItemsPerBlock) {
+                    bg.push_back((int)k);
+                    bf.push_back(f);
+                }
+            launch_frag_features(s_, A_, fdev, upload(b_fblk_g, bg), upload(b_fblk_f, bf),
+                                 (int)bg.size(), X);
             launched();
         }
         if (rank) {

@@ -135,8 +135,11 @@ struct FeatGroup {
     float prior;   // running best score of the pair before the stage
     float dense;   // 1 for the dense pass, 0 for the first pass
 };
-void launch_frag_features(hipStream_t s, const Arena &A, const FeatGroup *groups, int n_groups,
-                          long long n_rows, float *X);
+// Descriptor rows of all work items; block b covers items
+// [blk_first[b], blk_first[b] + 128) of group blk_group[b].
+constexpr int kFeatItemsPerBlock = 128;
+void launch_frag_features(hipStream_t s, const Arena &A, const FeatGroup *groups,
+                          const int *blk_group, const int *blk_first, int n_blocks, float *X);
 
 // C(m,n) = epi(sum_k A(m,k) B(k,n)) with A(m,k) = A[m*sam + k*sak],
 // B(k,n) = B[k*sbk + n*sbn], C(m,n) = C[m*scm + n*scn]; operands rounded to

@@ -4,7 +4,7 @@
 // a pair by two NW chains; almost all of them end below the pair's best.  The
 // learned method scores every work item with a small MLP over a 96-value
 // descriptor and refines only the top-m items of each pair.  This file holds
-//   - the descriptor kernel (one wavefront per work item),
+//   - the descriptor kernel (one lane per work item),
 //   - a BF16-operand / FP32-accumulate GEMM on the matrix cores
 //     (v_mfma_f32_32x32x8_bf16) that the MLP's forward and backward passes
 //     are made of, plus the small elementwise kernels of training (MSE, head
@@ -36,200 +36,160 @@ namespace {
 //     spaced fragment positions, 0.25 A bins (last bin open), sum 1.
 constexpr int kHistPos = 24;
 
-IA_D int find_feat_group(const FeatGroup *g, int n, long long row) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (g[mid].row0 <= row) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
+constexpr int kFeatBlock = kFeatItemsPerBlock;  // lanes per workgroup, all of one pair
 
-__global__ __launch_bounds__(256) void frag_feature_kernel(Arena A, const FeatGroup *groups,
-                                                           int n_groups, long long n_rows,
-                                                           float *X) {
-    __shared__ unsigned hist_s[4][64];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    unsigned *hist = hist_s[wv];
-    const long long stride = (long long)gridDim.x * 4;
-    for (long long row = (long long)blockIdx.x * 4 + wv; row < n_rows; row += stride) {
-        const FeatGroup &G = groups[find_feat_group(groups, n_groups, row)];
-        const PairInfo &pi = A.info[G.pair];
-        const int off = pi.off;
-        const int item = (int)(row - G.row0);
-        int L_ = 0;
-        while (L_ + 1 < G.n_lens && G.items_prefix[L_ + 1] <= item) L_++;
-        const int r = item - G.items_prefix[L_];
-        const int flen = G.lens[L_];
-        const int ps = (r / G.n_ns[L_]) * G.jp, ns = (r % G.n_ns[L_]) * G.jn;
-        double R[9], t[3];
-        wave_kabsch<true>(flen,
-                          [&](int q, double &x, double &y, double &z, double &a, double &b,
-                              double &c) {
-                              x = A.pcx[off + ps + q]; y = A.pcy[off + ps + q];
-                              z = A.pcz[off + ps + q];
-                              a = A.qcx[off + ns + q]; b = A.qcy[off + ns + q];
-                              c = A.qcz[off + ns + q];
-                          },
-                          R, t);
-        // fragment RMSD
-        double e = 0.0;
-        for (int q = lane; q < flen; q += 64)
-            e += dist2_moved(R, t, A.pcx[off + ps + q], A.pcy[off + ps + q], A.pcz[off + ps + q],
-                             A.qcx[off + ns + q], A.qcy[off + ns + q], A.qcz[off + ns + q]);
-        e = wave_sum(e);
-        // same-index TM-score and cutoff counts
-        const double d0sq = pi.d0 * pi.d0;
-        double tm = 0.0;
-        float cnt[kMaxCut];
-#pragma unroll
-        for (int c = 0; c < kMaxCut; c++) cnt[c] = 0.f;
-        for (int i = lane; i < pi.N; i += 64) {
-            if (!A.v[off + i]) continue;
-            const double d2 = dist2_moved(R, t, A.px[off + i], A.py[off + i], A.pz[off + i],
-                                          A.qx[off + i], A.qy[off + i], A.qz[off + i]);
-            tm += 1.0 / (1.0 + d2 / d0sq);
-#pragma unroll
-            for (int c = 0; c < kMaxCut; c++)
-                cnt[c] += (c < pi.C && d2 < pi.cut[c] * pi.cut[c]) ? 1.f : 0.f;
-        }
-        tm = wave_sum(tm);
-        // register pairing a <-> a + (ns - ps) over the compact rows
-        const int sh = ns - ps;
-        double tmr = 0.0;
-        int ov = 0;
-        for (int a = lane; a < pi.plen; a += 64) {
-            const int b = a + sh;
-            if (b < 0 || b >= pi.nlen) continue;
-            ov++;
-            const double d2 = dist2_moved(R, t, A.pcx[off + a], A.pcy[off + a], A.pcz[off + a],
-                                          A.qcx[off + b], A.qcy[off + b], A.qcz[off + b]);
-            tmr += 1.0 / (1.0 + d2 / d0sq);
-        }
-        tmr = wave_sum(tmr);
-        for (int m = 32; m > 0; m >>= 1) ov += __shfl_xor(ov, m, 64);
-        // two cutoff refinements of the same-index superposition (2 d0, d0)
-        double tmref[2] = {tm, tm};
-        {
-            double Rr[9], tr[3];
-#pragma unroll
-            for (int q = 0; q < 9; q++) Rr[q] = R[q];
-            tr[0] = t[0]; tr[1] = t[1]; tr[2] = t[2];
-            for (int step = 0; step < 2; step++) {
-                const double cut2 = step == 0 ? 4.0 * d0sq : d0sq;
-                double c[6] = {0, 0, 0, 0, 0, 0};
-                int n = 0;
-                for (int i = lane; i < pi.N; i += 64) {
-                    if (!A.v[off + i]) continue;
-                    const double x = A.px[off + i], y = A.py[off + i], z = A.pz[off + i];
-                    const double a = A.qx[off + i], b = A.qy[off + i], cc = A.qz[off + i];
-                    if (dist2_moved(Rr, tr, x, y, z, a, b, cc) >= cut2) continue;
-                    c[0] += x; c[1] += y; c[2] += z; c[3] += a; c[4] += b; c[5] += cc;
-                    n++;
-                }
-                for (int m = 32; m > 0; m >>= 1) n += __shfl_xor(n, m, 64);
-                if (n < 3) {
-                    tmref[step] = step == 0 ? tm : tmref[0];
-                    continue;
-                }
-#pragma unroll
-                for (int d = 0; d < 6; d++) c[d] = wave_sum(c[d]) / (double)n;
-                double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                for (int i = lane; i < pi.N; i += 64) {
-                    if (!A.v[off + i]) continue;
-                    const double x = A.px[off + i], y = A.py[off + i], z = A.pz[off + i];
-                    const double a = A.qx[off + i], b = A.qy[off + i], cc = A.qz[off + i];
-                    if (dist2_moved(Rr, tr, x, y, z, a, b, cc) >= cut2) continue;
-                    const double d0_ = x - c[0], d1_ = y - c[1], d2_ = z - c[2];
-                    const double e0 = a - c[3], e1 = b - c[4], e2 = cc - c[5];
-                    h[0] += d0_ * e0; h[1] += d0_ * e1; h[2] += d0_ * e2;
-                    h[3] += d1_ * e0; h[4] += d1_ * e1; h[5] += d1_ * e2;
-                    h[6] += d2_ * e0; h[7] += d2_ * e1; h[8] += d2_ * e2;
-                }
-#pragma unroll
-                for (int k = 0; k < 9; k++) h[k] = wave_sum(h[k]);
-                solve_rotation<true>(h, Rr);
-                const double cp[3] = {c[0], c[1], c[2]}, cq[3] = {c[3], c[4], c[5]};
-                finish_t(cp, cq, Rr, tr);
-                double sc = 0.0;
-                for (int i = lane; i < pi.N; i += 64) {
-                    if (!A.v[off + i]) continue;
-                    const double d2 = dist2_moved(Rr, tr, A.px[off + i], A.py[off + i],
-                                                  A.pz[off + i], A.qx[off + i], A.qy[off + i],
-                                                  A.qz[off + i]);
-                    sc += 1.0 / (1.0 + d2 / d0sq);
-                }
-                tmref[step] = wave_sum(sc);
-            }
-        }
-        // intra-fragment distance-difference histogram
-        hist[lane] = 0;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        const int m = flen < kHistPos ? flen : kHistPos;
-        for (int k1 = 0; k1 + 1 < m; k1++) {
-            const int k2 = k1 + 1 + lane;
-            if (k2 >= m) continue;
-            const int q1 = (k1 * (flen - 1)) / (m - 1), q2 = (k2 * (flen - 1)) / (m - 1);
-            const double ax = A.pcx[off + ps + q1] - A.pcx[off + ps + q2];
-            const double ay = A.pcy[off + ps + q1] - A.pcy[off + ps + q2];
-            const double az = A.pcz[off + ps + q1] - A.pcz[off + ps + q2];
-            const double bx = A.qcx[off + ns + q1] - A.qcx[off + ns + q2];
-            const double by = A.qcy[off + ns + q1] - A.qcy[off + ns + q2];
-            const double bz = A.qcz[off + ns + q1] - A.qcz[off + ns + q2];
-            const double dd = fabs(sqrt(ax * ax + ay * ay + az * az) -
-                                   sqrt(bx * bx + by * by + bz * bz));
-            const int bin = dd >= 15.75 ? 63 : (int)(dd * 4.0);
-            atomicAdd(&hist[bin], 1u);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        const int np = m * (m - 1) / 2;
-        const int ml = pi.plen < pi.nlen ? pi.plen : pi.nlen;
-        float tot[kMaxCut];
-#pragma unroll
-        for (int c = 0; c < kMaxCut; c++) {
-            float v = cnt[c];
-            for (int mm = 32; mm > 0; mm >>= 1) v += __shfl_xor(v, mm, 64);
-            tot[c] = v;
-        }
-        float *x = X + row * kFeat;
-        x[32 + lane] = np > 0 ? (float)hist[lane] / (float)np : 0.f;
-        if (lane < 32) {
-            float f = 0.f;
-            const double L = pi.Lnorm;
-            switch (lane) {
-                case 0: f = (float)flen / (float)ml; break;
-                case 1: f = (float)flen / 100.f; break;
-                case 2: f = (float)ps / (float)pi.plen; break;
-                case 3: f = (float)ns / (float)pi.nlen; break;
-                case 4: f = (float)(ps - ns) / (float)ml; break;
-                case 5: f = ps == ns ? 1.f : 0.f; break;
-                case 6: f = (float)(L / 300.0); break;
-                case 7: f = (float)(pi.d0 / 10.0); break;
-                case 8: f = G.dense; break;
-                case 9: f = G.prior; break;
-                case 10: f = (float)fmin(sqrt(e / flen) / pi.d0, 10.0); break;
-                case 11: f = (float)(tm / L); break;
-                case 12: f = (float)(tmr / L); break;
-                case 24: f = (float)pi.C / 11.f; break;
-                case 25: f = (float)pi.plen / 300.f; break;
-                case 26: f = (float)pi.nlen / 300.f; break;
-                case 27: f = (float)np / 276.f; break;
-                case 28: f = logf(1.f + (float)flen) / 5.f; break;
-                case 29: f = (float)ov / (float)ml; break;
-                case 30: f = (float)(tmref[0] / L); break;
-                case 31: f = (float)(tmref[1] / L); break;
-                default: break;
-            }
-#pragma unroll
-            for (int c = 0; c < kMaxCut; c++)
-                if (lane == 13 + c) f = (float)(tot[c] / L);
-            x[lane] = f;
-        }
-        __builtin_amdgcn_wave_barrier();
+// One lane = one work item; the lanes of a workgroup share the pair, so the
+// passes over the pair's residues are wave-uniform (scalar loads) and need
+// no cross-lane reductions.  Kabsch sums are single-pass FP64 moments;
+// TM-score terms are FP32.
+__global__ __launch_bounds__(kFeatBlock) void frag_feature_kernel(Arena A, const FeatGroup *groups,
+                                                                  const int *blk_group,
+                                                                  const int *blk_first, float *X) {
+    __shared__ unsigned hist_s[64 * kFeatBlock];
+    const FeatGroup &G = groups[blk_group[blockIdx.x]];
+    const int item = blk_first[blockIdx.x] + (int)threadIdx.x;
+    if (item >= G.items) return;
+    const PairInfo &pi = A.info[G.pair];
+    const int off = pi.off, N = pi.N;
+    int L_ = 0;
+    while (L_ + 1 < G.n_lens && G.items_prefix[L_ + 1] <= item) L_++;
+    const int r = item - G.items_prefix[L_];
+    const int flen = G.lens[L_];
+    const int ps = (r / G.n_ns[L_]) * G.jp, ns = (r % G.n_ns[L_]) * G.jn;
+    // fragment superposition and its RMSD from the moments
+    KabschSeq ks;
+    ks.reset();
+    double spp = 0.0, sqq = 0.0;
+    for (int q = 0; q < flen; q++) {
+        const P4 P = A.pc4[off + ps + q], Q = A.qc4[off + ns + q];
+        ks.add_moment(P.x, P.y, P.z, Q.x, Q.y, Q.z);
+        spp = fma(P.x, P.x, fma(P.y, P.y, fma(P.z, P.z, spp)));
+        sqq = fma(Q.x, Q.x, fma(Q.y, Q.y, fma(Q.z, Q.z, sqq)));
     }
+    ks.finish_moments();
+    double R[9], t[3];
+    ks.template solve<true>(R, t);
+    double rh = 0.0;
+#pragma unroll
+    for (int j = 0; j < 9; j++) rh = fma(R[j], ks.h[j], rh);
+    const double fl = (double)flen;
+    const double cpp = ks.cp[0] * ks.cp[0] + ks.cp[1] * ks.cp[1] + ks.cp[2] * ks.cp[2];
+    const double cqq = ks.cq[0] * ks.cq[0] + ks.cq[1] * ks.cq[1] + ks.cq[2] * ks.cq[2];
+    const double rmsd = sqrt(fmax(spp - fl * cpp + sqq - fl * cqq - 2.0 * rh, 0.0) / fl);
+    // same-index TM-score and cutoff counts
+    const double d0sq = pi.d0 * pi.d0;
+    const float inv_d0sq = (float)(1.0 / d0sq);
+    float cut2[kMaxCut], cnt[kMaxCut];
+#pragma unroll
+    for (int c = 0; c < kMaxCut; c++) {
+        cut2[c] = c < pi.C ? (float)(pi.cut[c] * pi.cut[c]) : -1.f;
+        cnt[c] = 0.f;
+    }
+    float tm = 0.f;
+    for (int i = 0; i < N; i++) {
+        const P4 P = A.p4[off + i];
+        if (P.w == 0.0) continue;
+        const P4 Q = A.q4[off + i];
+        const float d2 = (float)dist2_moved_fma(R, t, P.x, P.y, P.z, Q.x, Q.y, Q.z);
+        tm += __builtin_amdgcn_rcpf(__builtin_fmaf(d2, inv_d0sq, 1.f));
+#pragma unroll
+        for (int c = 0; c < kMaxCut; c++) cnt[c] += d2 < cut2[c] ? 1.f : 0.f;
+    }
+    // register pairing a <-> a + (ns - ps) over the compact rows
+    const int sh = ns - ps;
+    const int a0 = sh < 0 ? -sh : 0, a1 = min(pi.plen, pi.nlen - sh);
+    float tmr = 0.f;
+    for (int a = a0; a < a1; a++) {
+        const P4 P = A.pc4[off + a], Q = A.qc4[off + a + sh];
+        const float d2 = (float)dist2_moved_fma(R, t, P.x, P.y, P.z, Q.x, Q.y, Q.z);
+        tmr += __builtin_amdgcn_rcpf(__builtin_fmaf(d2, inv_d0sq, 1.f));
+    }
+    const int ov = a1 > a0 ? a1 - a0 : 0;
+    // two cutoff refinements of the same-index superposition (2 d0, d0)
+    float tmref[2] = {tm, tm};
+    for (int step = 0; step < 2; step++) {
+        const double cut = step == 0 ? 4.0 * d0sq : d0sq;
+        ks.reset();
+        for (int i = 0; i < N; i++) {
+            const P4 P = A.p4[off + i];
+            if (P.w == 0.0) continue;
+            const P4 Q = A.q4[off + i];
+            if (dist2_moved_fma(R, t, P.x, P.y, P.z, Q.x, Q.y, Q.z) < cut)
+                ks.add_moment(P.x, P.y, P.z, Q.x, Q.y, Q.z);
+        }
+        if (ks.cnt < 3.0) {
+            tmref[step] = step == 0 ? tm : tmref[0];
+            continue;
+        }
+        ks.finish_moments();
+        ks.template solve<true>(R, t);
+        float sc = 0.f;
+        for (int i = 0; i < N; i++) {
+            const P4 P = A.p4[off + i];
+            if (P.w == 0.0) continue;
+            const P4 Q = A.q4[off + i];
+            const float d2 = (float)dist2_moved_fma(R, t, P.x, P.y, P.z, Q.x, Q.y, Q.z);
+            sc += __builtin_amdgcn_rcpf(__builtin_fmaf(d2, inv_d0sq, 1.f));
+        }
+        tmref[step] = sc;
+    }
+    // intra-fragment distance-difference histogram (lane-private LDS bins)
+    unsigned *hist = hist_s + threadIdx.x;
+#pragma unroll 8
+    for (int b = 0; b < 64; b++) hist[b * kFeatBlock] = 0;
+    const int m = flen < kHistPos ? flen : kHistPos;
+    for (int k1 = 0; k1 + 1 < m; k1++) {
+        const int q1 = (k1 * (flen - 1)) / (m - 1);
+        const P4 P1 = A.pc4[off + ps + q1], Q1 = A.qc4[off + ns + q1];
+        for (int k2 = k1 + 1; k2 < m; k2++) {
+            const int q2 = (k2 * (flen - 1)) / (m - 1);
+            const P4 P2 = A.pc4[off + ps + q2], Q2 = A.qc4[off + ns + q2];
+            const float ax = (float)(P1.x - P2.x), ay = (float)(P1.y - P2.y), az = (float)(P1.z - P2.z);
+            const float bx = (float)(Q1.x - Q2.x), by = (float)(Q1.y - Q2.y), bz = (float)(Q1.z - Q2.z);
+            const float dd = fabsf(sqrtf(ax * ax + ay * ay + az * az) - sqrtf(bx * bx + by * by + bz * bz));
+            const int bin = dd >= 15.75f ? 63 : (int)(dd * 4.f);
+            hist[bin * kFeatBlock] += 1u;
+        }
+    }
+    const int np = m * (m - 1) / 2;
+    const int ml = pi.plen < pi.nlen ? pi.plen : pi.nlen;
+    const double L = pi.Lnorm;
+    float f[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) f[k] = 0.f;
+    f[0] = (float)flen / (float)ml;
+    f[1] = (float)flen / 100.f;
+    f[2] = (float)ps / (float)pi.plen;
+    f[3] = (float)ns / (float)pi.nlen;
+    f[4] = (float)(ps - ns) / (float)ml;
+    f[5] = ps == ns ? 1.f : 0.f;
+    f[6] = (float)(L / 300.0);
+    f[7] = (float)(pi.d0 / 10.0);
+    f[8] = G.dense;
+    f[9] = G.prior;
+    f[10] = (float)fmin(rmsd / pi.d0, 10.0);
+    f[11] = (float)(tm / L);
+    f[12] = (float)(tmr / L);
+#pragma unroll
+    for (int c = 0; c < kMaxCut; c++) f[13 + c] = (float)(cnt[c] / L);
+    f[24] = (float)pi.C / 11.f;
+    f[25] = (float)pi.plen / 300.f;
+    f[26] = (float)pi.nlen / 300.f;
+    f[27] = (float)np / 276.f;
+    f[28] = logf(1.f + (float)flen) / 5.f;
+    f[29] = (float)ov / (float)ml;
+    f[30] = (float)(tmref[0] / L);
+    f[31] = (float)(tmref[1] / L);
+    float4 *xo = (float4 *)(X + (G.row0 + item) * kFeat);
+#pragma unroll
+    for (int k = 0; k < 8; k++) xo[k] = make_float4(f[4 * k], f[4 * k + 1], f[4 * k + 2], f[4 * k + 3]);
+    const float inv_np = np > 0 ? 1.f / (float)np : 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+        xo[8 + k] = make_float4((float)hist[(4 * k) * kFeatBlock] * inv_np,
+                                (float)hist[(4 * k + 1) * kFeatBlock] * inv_np,
+                                (float)hist[(4 * k + 2) * kFeatBlock] * inv_np,
+                                (float)hist[(4 * k + 3) * kFeatBlock] * inv_np);
 }
 
 // ---- BF16 GEMM on the matrix cores --------------------------------------------
@@ -548,13 +508,11 @@ void launch_item_best(hipStream_t s, const Hit *out, long long n_items, float *y
                        out, n_items, y);
 }
 
-void launch_frag_features(hipStream_t s, const Arena &A, const FeatGroup *groups, int n_groups,
-                          long long n_rows, float *X) {
-    if (n_rows <= 0) return;
-    long long blocks = (n_rows + 3) / 4;
-    if (blocks > 256 * 64) blocks = 256 * 64;
-    hipLaunchKernelGGL(frag_feature_kernel, dim3((unsigned)blocks), dim3(256), 0, s, A, groups,
-                       n_groups, n_rows, X);
+void launch_frag_features(hipStream_t s, const Arena &A, const FeatGroup *groups,
+                          const int *blk_group, const int *blk_first, int n_blocks, float *X) {
+    if (n_blocks <= 0) return;
+    hipLaunchKernelGGL(frag_feature_kernel, dim3(n_blocks), dim3(kFeatBlock), 0, s, A, groups,
+                       blk_group, blk_first, X);
 }
 
 void launch_gemm(hipStream_t s, const GemmArgs &g) {
