@@ -91,7 +91,8 @@ int indialign_cuda_available(void);
  * still an FP64 TM-score of the returned transform); method = 2: learned
  * (method 1, and of the local-fragment rescue work items (rescue.h:60-118)
  * of each pair only the top-m by the ranker installed with
- * indialign_set_ranker are refined).  Same return codes as the batch entry,
+ * indialign_set_ranker are refined, and of the threading-DP offsets only
+ * the top ones by seed-bank score start NW chains).  Same return codes as the batch entry,
  * -3 for an unknown method, -4 for method 2 without a ranker. */
 int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
                         NativePairResult *outputs, int serial_bank, int method);
@@ -131,13 +132,18 @@ int indialign_last_stage_scores(double *out, int n_pairs);
 #define INDIALIGN_MLP_NFEAT 96
 #define INDIALIGN_MLP_NPARAM 90881
 
-/* Install (params != NULL) or remove (NULL) the ranker used by method 2, the
- * number of work items refined per pair and pass (top_m >= 1), and the
- * fallback margin: a pair whose largest predicted refined score reaches its
- * running best + fallback_margin refines every work item of that pass, as
- * the plain search does (fallback_margin < 0: never).  The parameters are
- * copied.  Returns 0, or -1 for top_m < 1. */
-int indialign_set_ranker(const float *params, int top_m, double fallback_margin);
+/* Install (params != NULL) or remove (NULL) the ranker used by method 2 and
+ * its budgets:
+ *   top_m >= 1: local-fragment work items refined per pair and pass;
+ *   fallback_margin: a pair whose largest predicted refined score reaches
+ *     its running best + fallback_margin refines every work item of that
+ *     pass, as the plain search does (< 0: never);
+ *   top_offsets >= 1: threading-DP offsets (rescue.h:121-190) per pair that
+ *     start NW chains, chosen by their seed-bank score (every offset still
+ *     runs its seed bank).
+ * The parameters are copied.  Returns 0, or -1 for a budget below 1. */
+int indialign_set_ranker(const float *params, int top_m, double fallback_margin,
+                         int top_offsets);
 
 /* Plain search (method 0, serial seed bank) that also records every
  * local-fragment work item: its descriptor row (X, INDIALIGN_MLP_NFEAT floats

@@ -132,6 +132,7 @@ class Engine {
     Mlp ranker;
     int top_m = 64;
     double fallback_margin = -1.0;  // < 0: never fall back to every work item
+    int top_offsets = 16;           // threading-DP offsets per pair that start chains
     // Record sink: when set, every local-fragment work item of the search
     // appends its descriptor row, its refined score (best of its two
     // chains) and its pair index, up to cap rows.
@@ -193,7 +194,7 @@ class Engine {
     // arena
     DBuf b_px, b_py, b_pz, b_qx, b_qy, b_qz, b_v, b_pv, b_nv, b_pcx, b_pcy, b_pcz, b_qcx, b_qcy,
         b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info, b_qfx, b_qfy, b_qfz, b_p4, b_q4, b_pc4,
-        b_qc4, b_ss_scr, b_fblk_g, b_fblk_f;
+        b_qc4, b_ss_scr, b_fblk_g, b_fblk_f, b_pq8, b_list_pq;
     // work buffers
     DBuf b_seeds, b_masks, b_banks, b_blk_bank, b_blk_first, b_cands, b_bank_out, b_lists_p,
         b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
@@ -204,7 +205,7 @@ class Engine {
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
                 &b_pcz, &b_qcx, &b_qcy, &b_qcz, &b_pmap, &b_nmap, &b_ssp, &b_ssn, &b_info,
-                &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4, &b_ss_scr, &b_fblk_g, &b_fblk_f,
+                &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4, &b_ss_scr, &b_fblk_g, &b_fblk_f, &b_pq8, &b_list_pq,
                 &b_seeds, &b_masks, &b_banks, &b_blk_bank, &b_blk_first, &b_cands,
                 &b_bank_out, &b_lists_p, &b_lists_n, &b_list_na, &b_lout, &b_groups, &b_xf,
                 &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
@@ -354,6 +355,9 @@ void Engine::prepare(const NativePairInput *inp, int n) {
             pc4[i] = P4{pcx[i], pcy[i], pcz[i], 0.0};
             qc4[i] = P4{qcx[i], qcy[i], qcz[i], 0.0};
         }
+        std::vector<P8> pq8(px.size());
+        for (size_t i = 0; i < px.size(); i++) pq8[i] = P8{p4[i], q4[i]};
+        A_.pq8 = upload(b_pq8, pq8);
         A_.p4 = upload(b_p4, p4);
         A_.q4 = upload(b_q4, q4);
         A_.pc4 = upload(b_pc4, pc4);
@@ -425,10 +429,18 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
     const int *dbf = upload(b_blk_first, blk_first);
     const SeedDesc *ds = upload(b_seeds, seeds);
     Hit *dc = b_cands.get<Hit>(cand);
+    int max_slot = -1;
+    for (const BankDesc &b : banks)
+        if (b.pts == PTS_LIST) max_slot = std::max(max_slot, b.a0);
+    P8 *lpq = b_list_pq.get<P8>((size_t)(max_slot + 1) * list_cap_);
+    if (max_slot >= 0) {
+        launch_list_gather(s_, A_, db, (int)banks.size(), b_lists_p.get<int>(1),
+                           b_lists_n.get<int>(1), b_list_na.get<int>(1), list_cap_, lpq);
+        launched();
+    }
     IA_CHECK(hipEventRecord(e0_, s_));
-    launch_bank(s_, A_, db, dbb, dbf, (int)blk_bank.size(), ds, b_masks.get<uint8_t>(1),
-                b_lists_p.get<int>(1), b_lists_n.get<int>(1), b_list_na.get<int>(1), list_cap_,
-                dc, fast_);
+    launch_bank(s_, A_, db, dbb, dbf, (int)blk_bank.size(), ds, b_masks.get<uint8_t>(1), lpq,
+                b_list_na.get<int>(1), list_cap_, dc, fast_);
     launched();
     IA_CHECK(hipGetLastError());
     IA_CHECK(hipEventRecord(e1_, s_));
@@ -1047,7 +1059,40 @@ void Engine::stage_rescue() {
                 base += 3LL * no;
             }
             Hit *dthr = b_chain_out.get<Hit>(base);
+            // learned method: only the top_offsets offsets of each pair by
+            // their bank score start NW chains; the other chain slots keep
+            // "no result"
+            const bool prune = learned_ && top_offsets > 0 && !groups.empty();
+            if (prune) {
+                launch_fill_none(s_, dthr, base);
+                launched();
+            }
             run_banks(banks, {}, base, dthr);
+            if (prune) {
+                const long long rows = base / 3;
+                float *sc = b_pred.get<float>(rows);
+                launch_hit_scores(s_, dthr, rows, 3, sc);
+                std::vector<long long> row0(groups.size()), sel_off(groups.size());
+                std::vector<int> items(groups.size());
+                long long ns = 0;
+                for (size_t k = 0; k < groups.size(); k++) {
+                    row0[k] = groups[k].xf_off / 3;
+                    items[k] = groups[k].n_chains / 2;
+                    sel_off[k] = ns;
+                    ns += std::min(items[k], top_offsets);
+                }
+                int *isel = b_isel.get<int>(ns);
+                launch_select_top(s_, sc, upload(b_srow, row0), upload(b_sitems, items),
+                                  (int)groups.size(), top_offsets, isel, upload(b_ssel, sel_off),
+                                  b_scnt.get<int>(groups.size()), b_pmax.get<float>(groups.size()));
+                launched();
+                A_.isel = isel;
+                for (size_t k = 0; k < groups.size(); k++) {
+                    groups[k].sel = 1;
+                    groups[k].sel_off = sel_off[k];
+                    groups[k].n_chains = 2 * std::min(items[k], top_offsets);
+                }
+            }
             run_chains(groups, dthr, dthr);
             const std::vector<Hit> red = group_reduce(offs, cnts, dthr);
             const std::vector<Hit> rs = rescore(P, red);
@@ -1288,6 +1333,7 @@ double g_stats[8];
 std::vector<float> g_ranker;
 int g_top_m = 64;
 double g_margin = -1.0;
+int g_top_offsets = 16;
 unsigned g_ranker_ver = 0;
 unsigned g_engine_ranker_ver = 0;
 
@@ -1303,6 +1349,7 @@ void sync_ranker() {
     else g_engine->ranker.set_params(g_engine->stream(), g_ranker.data());
     g_engine->top_m = g_top_m;
     g_engine->fallback_margin = g_margin;
+    g_engine->top_offsets = g_top_offsets;
     g_engine_ranker_ver = g_ranker_ver;
 }
 
@@ -1441,12 +1488,14 @@ int indialign_last_stage_scores(double *out, int n_pairs) {
     return m;
 }
 
-int indialign_set_ranker(const float *params, int top_m, double fallback_margin) {
-    if (params && top_m < 1) return -1;
+int indialign_set_ranker(const float *params, int top_m, double fallback_margin,
+                         int top_offsets) {
+    if (params && (top_m < 1 || top_offsets < 1)) return -1;
     std::lock_guard<std::mutex> lk(g_mu);
     if (params) g_ranker.assign(params, params + ia::kNParam);
     else g_ranker.clear();
     g_top_m = params ? top_m : 64;
+    g_top_offsets = params ? top_offsets : 16;
     g_margin = fallback_margin;
     g_ranker_ver++;
     return 0;

@@ -43,6 +43,11 @@ struct alignas(32) P4 {
     double x, y, z, w;
 };
 
+// A point pair, 64 bytes: one wide load for both sides.
+struct alignas(64) P8 {
+    P4 p, q;
+};
+
 struct Arena {
     const double *px, *py, *pz, *qx, *qy, *qz;
     const uint8_t *v, *pv, *nv;
@@ -54,6 +59,7 @@ struct Arena {
     const int *isel;                                   // selected work items (learned method)
     const P4 *p4, *q4;                                 // full-index AoS copies (p4.w = valid)
     const P4 *pc4, *qc4;                               // compact AoS copies
+    const P8 *pq8;                                     // full-index pairs (pred, native)
 };
 
 IA_HD void set_identity(double *R, double *t) {

@@ -46,6 +46,8 @@ enum ScoreMode { SM_TM = 0, SM_SS = 1, SM_IA4 = 2 };
 //              of the fragment pair (rescue.h:84-91).  With sel != 0 the
 //              chain's item is A.isel[sel_off + i] (a selected subset, in
 //              increasing item order) and results stay at index i.
+//              CI_XFORM with sel != 0: chain i runs item A.isel[sel_off + i]
+//              in place (its start and its result slots).
 //   CI_OFFSET: offset k = k0 + (c/2)*kstep; start = Kabsch of the pairing
 //              pred[i] <-> native[i-k] (indialign.cpp:234-247); k == 0 or
 //              fewer than 3 pairs -> the chain reports "skipped".
@@ -84,8 +86,13 @@ struct ListOut {
 // Launchers (defined in the .hip files; all asynchronous on `s`).
 void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
                  const int *blk_first, int n_blocks, const SeedDesc *seeds,
-                 const uint8_t *masks, const int *lists_p, const int *lists_n,
-                 const int *list_na, int list_cap, Hit *cands, bool fast);
+                 const uint8_t *masks, const P8 *list_pq, const int *list_na, int list_cap,
+                 Hit *cands, bool fast);
+// Interleaved copies of the traceback lists of the PTS_LIST banks, read by
+// launch_bank (list_pq[slot * list_cap + k]).
+void launch_list_gather(hipStream_t s, const Arena &A, const BankDesc *banks, int n_banks,
+                        const int *lists_p, const int *lists_n, const int *list_na, int list_cap,
+                        P8 *list_pq);
 void launch_bank_reduce(hipStream_t s, const BankDesc *banks, int n_banks, const int *list_na,
                         const Hit *cands, Hit *out);
 void launch_group_reduce(hipStream_t s, const long long *offs, const int *counts, int n,
@@ -175,6 +182,10 @@ void launch_select_top(hipStream_t s, const float *pred, const long long *row0,
                        const int *items, int n_groups, int m, int *sel, const long long *sel_off,
                        int *count, float *pmax);
 
+// out[i] = "no result" (score -inf, identity) for i < n.
+void launch_fill_none(hipStream_t s, Hit *out, long long n);
+// y[i] = (float) in[i * stride].score for i < n.
+void launch_hit_scores(hipStream_t s, const Hit *in, long long n, int stride, float *y);
 // y[i] = best score of the two chains of work item i (out[2i], out[2i+1]).
 void launch_item_best(hipStream_t s, const Hit *out, long long n_items, float *y);
 // dst[r] = src[idx[r]] (rows of kFeat values), tdst[r] = tsrc[idx[r]]

@@ -14,19 +14,22 @@ namespace ia {
 
 namespace {
 
-// Point i of the bank's point set: one 32-byte load per side (wave-uniform
-// address, so a scalar load).  Returns false for an invalid residue
-// (PTS_FULL only).
+// Point i of the bank's point set: one 64-byte load for full-index and list
+// points (a pre-gathered list copy), two 32-byte loads for compact windows;
+// the address is wave-uniform, so these are scalar loads.  Returns false for
+// an invalid residue (PTS_FULL only).
 template <int PTS>
-IA_D bool get_pt(const Arena &A, int off, const int *lp, const int *ln, int a0, int a1, int i,
-                 double &x, double &y, double &z, double &qa, double &qb, double &qc) {
+IA_D bool get_pt(const Arena &A, int off, const P8 *lq, int a0, int a1, int i, double &x,
+                 double &y, double &z, double &qa, double &qb, double &qc) {
     P4 p, q;
     if (PTS == PTS_FULL) {
-        p = A.p4[off + i];
-        q = A.q4[off + i];
+        const P8 r = A.pq8[off + i];
+        p = r.p;
+        q = r.q;
     } else if (PTS == PTS_LIST) {
-        p = A.p4[off + lp[i]];
-        q = A.q4[off + ln[i]];
+        const P8 r = lq[i];
+        p = r.p;
+        q = r.q;
     } else {
         p = A.pc4[off + a0 + i];
         q = A.qc4[off + a1 + i];
@@ -49,7 +52,7 @@ IA_D int bank_seeds(const BankDesc &B, int n) {
 // TM-score of search.h:358-362.
 template <int PTS, bool FAST>
 IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const SeedDesc *seeds,
-                        const uint8_t *masks, const int *lp, const int *ln, Hit &out) {
+                        const uint8_t *masks, const P8 *lq, Hit &out) {
     const PairInfo &pi = A.info[B.pair];
     const int off = pi.off;
     const int C = B.C;
@@ -104,7 +107,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
             const bool in = win ? (i >= sd.a && i < sd.a + sd.b) : (m[i] != 0);
             if (!in) continue;
             double x, y, z, qa, qb, qc;
-            if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+            if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
             if (FAST) ks.add_moment(x, y, z, qa, qb, qc);
             else ks.add_centroid(x, y, z, qa, qb, qc);
         }
@@ -119,7 +122,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
                 const bool in = win ? (i >= sd.a && i < sd.a + sd.b) : (m[i] != 0);
                 if (!in) continue;
                 double x, y, z, qa, qb, qc;
-                if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+                if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
                 ks.add_cov(x, y, z, qa, qb, qc);
             }
             ks.template solve<FAST>(R, t);
@@ -135,7 +138,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         if (FAST) {  // one pass: selection + raw moments
             for (int i = 0; i < n; i++) {
                 double x, y, z, qa, qb, qc;
-                if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+                if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
                 if (dist2_moved_fma(R, t, x, y, z, qa, qb, qc) <= lim) {
                     cnt++;
                     ks.add_moment(x, y, z, qa, qb, qc);
@@ -151,7 +154,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         }
         for (int i = 0; i < n; i++) {
             double x, y, z, qa, qb, qc;
-            if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+            if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
             if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
                 cnt++;
                 ks.add_centroid(x, y, z, qa, qb, qc);
@@ -161,7 +164,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
             ks.centre();
             for (int i = 0; i < n; i++) {
                 double x, y, z, qa, qb, qc;
-                if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+                if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
                 if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) ks.add_cov(x, y, z, qa, qb, qc);
             }
             ks.template solve<FAST>(R, t);
@@ -174,7 +177,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
     double acc = 0.0;
     for (int i = 0; i < n; i++) {
         double x, y, z, qa, qb, qc;
-        if (!get_pt<PTS>(A, off, lp, ln, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+        if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
         const double d2 = FAST ? dist2_moved_fma(R, t, x, y, z, qa, qb, qc)
                                : dist2_moved(R, t, x, y, z, qa, qb, qc);
         if (B.use_d8 && !(d2 <= c8)) continue;
@@ -191,9 +194,8 @@ template <bool FAST>
 __global__ __launch_bounds__(256) void bank_kernel(Arena A, const BankDesc *banks,
                                                    const int *blk_bank, const int *blk_first,
                                                    const SeedDesc *seeds, const uint8_t *masks,
-                                                   const int *lists_p, const int *lists_n,
-                                                   const int *list_na, int list_cap,
-                                                   Hit *cands) {
+                                                   const P8 *list_pq, const int *list_na,
+                                                   int list_cap, Hit *cands) {
     const BankDesc &B = banks[blk_bank[blockIdx.x]];
     const int j = blk_first[blockIdx.x] + (int)threadIdx.x;
     if (j >= B.cand_cap) return;
@@ -204,14 +206,33 @@ __global__ __launch_bounds__(256) void bank_kernel(Arena A, const BankDesc *bank
         h.score = -INFINITY;
         set_identity(h.R, h.t);
     } else if (B.pts == PTS_FULL) {
-        run_candidate<PTS_FULL, FAST>(A, B, n, j, seeds, masks, nullptr, nullptr, h);
+        run_candidate<PTS_FULL, FAST>(A, B, n, j, seeds, masks, nullptr, h);
     } else if (B.pts == PTS_LIST) {
         const long long base = (long long)B.a0 * list_cap;
-        run_candidate<PTS_LIST, FAST>(A, B, n, j, seeds, masks, lists_p + base, lists_n + base, h);
+        run_candidate<PTS_LIST, FAST>(A, B, n, j, seeds, masks, list_pq + base, h);
     } else {
-        run_candidate<PTS_CWIN, FAST>(A, B, n, j, seeds, masks, nullptr, nullptr, h);
+        run_candidate<PTS_CWIN, FAST>(A, B, n, j, seeds, masks, nullptr, h);
     }
     cands[B.cand_off + j] = h;
+}
+
+// Copy the aligned pairs of every PTS_LIST bank's traceback list into
+// interleaved 64-byte records (list_pq[slot * list_cap + k]).
+__global__ __launch_bounds__(256) void list_gather_kernel(Arena A, const BankDesc *banks,
+                                                          const int *lists_p, const int *lists_n,
+                                                          const int *list_na, int list_cap,
+                                                          P8 *list_pq) {
+    const BankDesc &B = banks[blockIdx.x];
+    if (B.pts != PTS_LIST) return;
+    const int off = A.info[B.pair].off;
+    const long long base = (long long)B.a0 * list_cap;
+    const int na = list_na[B.a0];
+    for (int k = threadIdx.x; k < na; k += 256) {
+        P8 r;
+        r.p = A.p4[off + lists_p[base + k]];
+        r.q = A.q4[off + lists_n[base + k]];
+        list_pq[base + k] = r;
+    }
 }
 
 // (score, index) first-wins maximum over a block.
@@ -410,15 +431,23 @@ __global__ __launch_bounds__(64) void weighted_kernel(Arena A, const int *pairs,
 
 void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
                  const int *blk_first, int n_blocks, const SeedDesc *seeds,
-                 const uint8_t *masks, const int *lists_p, const int *lists_n,
-                 const int *list_na, int list_cap, Hit *cands, bool fast) {
+                 const uint8_t *masks, const P8 *list_pq, const int *list_na, int list_cap,
+                 Hit *cands, bool fast) {
     if (n_blocks <= 0) return;
     if (fast)
         bank_kernel<true><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks,
-                                                   lists_p, lists_n, list_na, list_cap, cands);
+                                                   list_pq, list_na, list_cap, cands);
     else
         bank_kernel<false><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks,
-                                                    lists_p, lists_n, list_na, list_cap, cands);
+                                                    list_pq, list_na, list_cap, cands);
+}
+
+void launch_list_gather(hipStream_t s, const Arena &A, const BankDesc *banks, int n_banks,
+                        const int *lists_p, const int *lists_n, const int *list_na, int list_cap,
+                        P8 *list_pq) {
+    if (n_banks <= 0) return;
+    list_gather_kernel<<<n_banks, 256, 0, s>>>(A, banks, lists_p, lists_n, list_na, list_cap,
+                                               list_pq);
 }
 
 void launch_bank_reduce(hipStream_t s, const BankDesc *banks, int n_banks, const int *list_na,

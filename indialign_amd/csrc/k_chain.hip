@@ -435,11 +435,14 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     const double gap = tm ? ((c & 1) ? 0.0 : -0.6) : G.gap_ss;
     const int item = tm ? (c >> 1) : c;
     const int g = tm ? (c & 1) : 0;
-    Hit *dst = out + G.out_off + (long long)item * G.out_stride + g;
+    // CI_XFORM with a selection: the chain of selected item i runs item
+    // isel[sel_off + i] in place (start and result slots of that item)
+    const int xitem = (G.init == CI_XFORM && G.sel) ? A.isel[G.sel_off + item] : item;
+    Hit *dst = out + G.out_off + (long long)xitem * G.out_stride + g;
     double R[9], t[3];
     bool skip = false;
     if (G.init == CI_XFORM) {
-        const Hit &h = xf[G.xf_off + (long long)item * G.xf_stride + (G.xf_per_chain ? g : 0)];
+        const Hit &h = xf[G.xf_off + (long long)xitem * G.xf_stride + (G.xf_per_chain ? g : 0)];
 #pragma unroll
         for (int q = 0; q < 9; q++) R[q] = h.R[q];
         t[0] = h.t[0]; t[1] = h.t[1]; t[2] = h.t[2];

@@ -493,6 +493,21 @@ __global__ void gather_rows_kernel(const float *src, const float *tsrc, const in
     if (c == 0) tdst[r] = tsrc[s];
 }
 
+__global__ void fill_none_kernel(Hit *out, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Hit h;
+    set_identity(h.R, h.t);
+    h.score = -INFINITY;
+    out[i] = h;
+}
+
+__global__ void hit_scores_kernel(const Hit *in, long long n, int stride, float *y) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = (float)in[i * stride].score;
+}
+
 __global__ void item_best_kernel(const Hit *out, long long n, float *y) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -501,6 +516,17 @@ __global__ void item_best_kernel(const Hit *out, long long n, float *y) {
 }
 
 }  // namespace
+
+void launch_fill_none(hipStream_t s, Hit *out, long long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(fill_none_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, n);
+}
+
+void launch_hit_scores(hipStream_t s, const Hit *in, long long n, int stride, float *y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(hit_scores_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in,
+                       n, stride, y);
+}
 
 void launch_item_best(hipStream_t s, const Hit *out, long long n_items, float *y) {
     if (n_items <= 0) return;

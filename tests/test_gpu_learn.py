@@ -100,12 +100,12 @@ def test_mlp_training_fits_a_known_function():
 
 
 def test_learned_unlimited_budget_is_fast_method():
-    """top_m above every pair's item count: the selection keeps all items in
-    order, so method 2 must reproduce method 1 bit for bit."""
+    """Budgets above every pair's item and offset counts: the selections keep
+    everything in order, so method 2 must reproduce method 1 bit for bit."""
     _gpu_or_skip()
     pairs = synth.short_pairs(24, first_seed=501)
     sf, Rf, tf = ia.tmscore_search_batch(pairs, serial_bank=1, method=ia.METHOD_FAST)
-    ia.set_ranker(ia.init_params(0), top_m=1 << 30)
+    ia.set_ranker(ia.init_params(0), top_m=1 << 30, top_offsets=1 << 30)
     sl, Rl, tl = ia.tmscore_search_batch(pairs, serial_bank=1, method=ia.METHOD_LEARNED)
     ia.set_ranker(None)
     np.testing.assert_array_equal(sl, sf)
