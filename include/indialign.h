@@ -124,6 +124,18 @@ int indialign_last_stats(double *out, int n);
  * written. */
 int indialign_last_stage_scores(double *out, int n_pairs);
 
+/* The rotation solvers of the search, run on the host (the device code is
+ * the same source): from the 3x3 covariance h[3k+d] = sum (p_k - cp_k)
+ * (q_d - cq_d), the rotation R maximizing sum (R p') . q' (out_d =
+ * sum_k p_k R[3k+d]).  method 0: Horn key matrix + 8-sweep Jacobi, the
+ * reference's solver (core.h:64-123, the plain method); method 1: the
+ * closed-form quaternion solve of the fast method (largest key-matrix
+ * eigenvalue by Newton on its characteristic quartic, eigenvector from the
+ * adjugate).  Returns 0; for method 1, 1 when the closed form declines (near
+ * repeated top eigenvalue; the fast method then uses method 0); -1 for an
+ * unknown method. */
+int indialign_rotation_from_covariance(const double *h, int method, double *R);
+
 /* ---- learned ranking of the local-fragment work items ---------------------
  * The ranker is an MLP 96 -> 256 -> 256 -> 1 (ReLU) over a work-item
  * descriptor; its parameters are one float array of INDIALIGN_MLP_NPARAM

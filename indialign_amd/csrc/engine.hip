@@ -1479,6 +1479,15 @@ int indialign_last_stats(double *out, int n) {
     return m;
 }
 
+int indialign_rotation_from_covariance(const double *h, int method, double *R) {
+    if (method == 0) {
+        ia::horn_rotation(h, R);
+        return 0;
+    }
+    if (method == 1) return ia::qcp_rotation(h, R) ? 0 : 1;
+    return -1;
+}
+
 int indialign_last_stage_scores(double *out, int n_pairs) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_engine || n_pairs <= 0) return 0;
