@@ -91,8 +91,9 @@ int indialign_cuda_available(void);
  * still an FP64 TM-score of the returned transform); method = 2: learned
  * (method 1, and of the local-fragment rescue work items (rescue.h:60-118)
  * of each pair only the top-m by the ranker installed with
- * indialign_set_ranker are refined, and of the threading-DP offsets only
- * the top ones by seed-bank score start NW chains).  Same return codes as the batch entry,
+ * indialign_set_ranker are refined; the threading-DP offsets and the seeds
+ * of the nested detailed searches are screened the same way, see
+ * indialign_set_learned_budgets).  Same return codes as the batch entry,
  * -3 for an unknown method, -4 for method 2 without a ranker. */
 int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
                         NativePairResult *outputs, int serial_bank, int method);
@@ -144,18 +145,27 @@ int indialign_rotation_from_covariance(const double *h, int method, double *R);
 #define INDIALIGN_MLP_NFEAT 96
 #define INDIALIGN_MLP_NPARAM 90881
 
-/* Install (params != NULL) or remove (NULL) the ranker used by method 2 and
- * its budgets:
- *   top_m >= 1: local-fragment work items refined per pair and pass;
- *   fallback_margin: a pair whose largest predicted refined score reaches
- *     its running best + fallback_margin refines every work item of that
- *     pass, as the plain search does (< 0: never);
+/* Install (params != NULL, copied) or remove (NULL) the ranker used by
+ * method 2.  Returns 0. */
+int indialign_set_ranker(const float *params);
+
+/* Work budgets of method 2 (defaults 64, 16, 0, -1):
+ *   top_m >= 1: local-fragment work items refined per pair and pass, the
+ *     ranker's top ones;
  *   top_offsets >= 1: threading-DP offsets (rescue.h:121-190) per pair that
  *     start NW chains, chosen by their seed-bank score (every offset still
- *     runs its seed bank).
- * The parameters are copied.  Returns 0, or -1 for a budget below 1. */
-int indialign_set_ranker(const float *params, int top_m, double fallback_margin,
-                         int top_offsets);
+ *     runs its seed bank);
+ *   top_seeds >= 0: seeds refined per bank of the SS-seed stage (its
+ *     detailed searches and DP refinement, ss.h:56-164) and of threading DP,
+ *     chosen by the TM-score of each seed's initial superposition; 0 refines
+ *     every seed (the main DP refinement, indialign.cpp:179-223, always
+ *     refines every seed);
+ *   fallback_margin: a pair whose largest predicted refined score reaches
+ *     its running best + fallback_margin refines every local-fragment work
+ *     item of that pass, as the plain search does (< 0: never).
+ * Returns 0, or -1 for a budget out of range. */
+int indialign_set_learned_budgets(int top_m, int top_offsets, int top_seeds,
+                                  double fallback_margin);
 
 /* Plain search (method 0, serial seed bank) that also records every
  * local-fragment work item: its descriptor row (X, INDIALIGN_MLP_NFEAT floats

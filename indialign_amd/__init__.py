@@ -88,8 +88,11 @@ def load_library(path: str | None = None):
     c_float_p = ctypes.POINTER(ctypes.c_float)
     lib.indialign_last_stage_scores.argtypes = [c_double_p, ctypes.c_int]
     lib.indialign_last_stage_scores.restype = ctypes.c_int
-    lib.indialign_set_ranker.argtypes = [c_float_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.indialign_set_ranker.argtypes = [c_float_p]
     lib.indialign_set_ranker.restype = ctypes.c_int
+    lib.indialign_set_learned_budgets.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_double]
+    lib.indialign_set_learned_budgets.restype = ctypes.c_int
     lib.indialign_collect_records.argtypes = [
         ctypes.c_int, ctypes.POINTER(NativePairInput), ctypes.POINTER(NativeConfig),
         ctypes.POINTER(NativePairResult), c_float_p, c_float_p, ctypes.POINTER(ctypes.c_int),
@@ -256,21 +259,25 @@ def _f32(a) -> np.ndarray:
 
 
 def set_ranker(params, top_m: int = 64, fallback_margin: float = -1.0,
-               top_offsets: int = 16) -> None:
+               top_offsets: int = 16, top_seeds: int = 0) -> None:
     """Install the ranker (NPARAM float32 values) used by METHOD_LEARNED, or
-    remove it with params=None.  top_m: local-fragment work items refined per
-    pair and pass; fallback_margin >= 0: a pair whose largest predicted
-    refined score reaches its running best + margin refines every work item
-    of that pass; top_offsets: threading-DP offsets per pair that start NW
-    chains (by seed-bank score)."""
+    remove it with params=None, and set the method's budgets
+    (indialign_set_learned_budgets): top_m local-fragment work items per pair
+    and pass, top_offsets threading-DP offsets per pair that start NW chains,
+    top_seeds seeds refined per nested detailed-search bank (0 = all), and
+    the fallback margin (< 0: off)."""
+    lib = load_library()
     if params is None:
-        load_library().indialign_set_ranker(None, 64, -1.0, 16)
+        lib.indialign_set_ranker(None)
         return
     p = _f32(params).reshape(-1)
     if p.size != NPARAM:
         raise ValueError(f"ranker needs {NPARAM} parameters, got {p.size}")
-    rc = load_library().indialign_set_ranker(p.ctypes.data_as(_f32p), int(top_m),
-                                             float(fallback_margin), int(top_offsets))
+    rc = lib.indialign_set_learned_budgets(int(top_m), int(top_offsets), int(top_seeds),
+                                           float(fallback_margin))
+    if rc != 0:
+        raise IndialignError(f"indialign_set_learned_budgets returned {rc}")
+    rc = lib.indialign_set_ranker(p.ctypes.data_as(_f32p))
     if rc != 0:
         raise IndialignError(f"indialign_set_ranker returned {rc}")
 
@@ -355,8 +362,9 @@ DEFAULT_RANKER = os.path.join(HERE, "ranker_params.npy")
 
 
 def load_default_ranker(top_m: int = 64, fallback_margin: float = -1.0,
-                        top_offsets: int = 16, path: str | None = None) -> np.ndarray:
+                        top_offsets: int = 16, top_seeds: int = 0,
+                        path: str | None = None) -> np.ndarray:
     """Install the packaged trained ranker (train_ranker.py) for METHOD_LEARNED."""
     p = np.load(path or DEFAULT_RANKER, allow_pickle=False)
-    set_ranker(p, top_m, fallback_margin, top_offsets)
+    set_ranker(p, top_m, fallback_margin, top_offsets, top_seeds)
     return p

@@ -133,6 +133,7 @@ class Engine {
     int top_m = 64;
     double fallback_margin = -1.0;  // < 0: never fall back to every work item
     int top_offsets = 16;           // threading-DP offsets per pair that start chains
+    int top_seeds = 0;              // seeds refined per SS-stage and threading bank (0 = all)
     // Record sink: when set, every local-fragment work item of the search
     // appends its descriptor row, its refined score (best of its two
     // chains) and its pair index, up to cap rows.
@@ -194,7 +195,8 @@ class Engine {
     // arena
     DBuf b_px, b_py, b_pz, b_qx, b_qy, b_qz, b_v, b_pv, b_nv, b_pcx, b_pcy, b_pcz, b_qcx, b_qcy,
         b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info, b_qfx, b_qfy, b_qfz, b_p4, b_q4, b_pc4,
-        b_qc4, b_ss_scr, b_fblk_g, b_fblk_f, b_pq8, b_list_pq;
+        b_qc4, b_ss_scr, b_fblk_g, b_fblk_f, b_pq8, b_list_pq, b_sbanks, b_sblk_bank,
+        b_sblk_first, b_scands;
     // work buffers
     DBuf b_seeds, b_masks, b_banks, b_blk_bank, b_blk_first, b_cands, b_bank_out, b_lists_p,
         b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
@@ -205,7 +207,8 @@ class Engine {
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
                 &b_pcz, &b_qcx, &b_qcy, &b_qcz, &b_pmap, &b_nmap, &b_ssp, &b_ssn, &b_info,
-                &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4, &b_ss_scr, &b_fblk_g, &b_fblk_f, &b_pq8, &b_list_pq,
+                &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4, &b_ss_scr, &b_fblk_g, &b_fblk_f, &b_pq8, &b_list_pq, &b_sbanks, &b_sblk_bank, &b_sblk_first,
+                &b_scands,
                 &b_seeds, &b_masks, &b_banks, &b_blk_bank, &b_blk_first, &b_cands,
                 &b_bank_out, &b_lists_p, &b_lists_n, &b_list_na, &b_lout, &b_groups, &b_xf,
                 &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
@@ -241,8 +244,12 @@ class Engine {
     void stage_weighted();
 
     // Evaluate banks; returns device results indexed by out_idx (n_out).
+    // screen > 0: first score every seed's initial superposition (one
+    // cutoff, no refinement), then refine only each bank's top `screen`
+    // seeds at every cutoff.
     Hit *run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc> &seeds,
-                   size_t n_out, Hit *out_dev = nullptr);
+                   size_t n_out, Hit *out_dev = nullptr, int screen = 0);
+    void screen_seeds(std::vector<BankDesc> &banks, const SeedDesc *seeds_dev, int screen);
     // Launch chain groups (chain_off filled here); returns number of chains.
     long long run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev, Hit *out_dev);
     std::vector<Hit> rescore(const std::vector<int> &pairs, const std::vector<Hit> &in);
@@ -251,7 +258,7 @@ class Engine {
     // dp_refine (align.h:149-159) for a list of (pair, start); lite selects the
     // fused one-launch path, non-lite the per-iteration list + bank path.
     std::vector<Hit> dp_refine(const std::vector<int> &pairs, const std::vector<Hit> &start,
-                               int iters, bool lite);
+                               int iters, bool lite, int screen = 0);
     void offer(int a, const Hit &h) {
         if (h.score > best_[a].score) best_[a] = h;
     }
@@ -409,8 +416,73 @@ std::vector<int> Engine::max_seed_table(int nmax, int ms) {
     return t;
 }
 
+void Engine::screen_seeds(std::vector<BankDesc> &banks, const SeedDesc *seeds_dev, int screen) {
+    std::vector<BankDesc> sb(banks);
+    std::vector<int> blk_bank, blk_first;
+    std::vector<long long> row0(banks.size());
+    long long cand = 0;
+    for (size_t b = 0; b < sb.size(); b++) {
+        BankDesc &x = sb[b];
+        x.cand_cap = banks[b].cand_cap / std::max(banks[b].C, 1);  // seeds (upper bound)
+        x.C = 1;
+        x.iters = 0;
+        x.et = 1e300;
+        x.sel_m = 0;
+        x.cand_off = cand;
+        row0[b] = cand;
+        cand += x.cand_cap;
+        for (int j = 0; j < x.cand_cap; j += 256) {
+            blk_bank.push_back((int)b);
+            blk_first.push_back(j);
+        }
+    }
+    if (cand == 0) return;
+    int kmax = 1;  // a budget above every bank's seed count keeps them all
+    for (const BankDesc &x : sb) kmax = std::max(kmax, x.cand_cap);
+    screen = std::min(screen, kmax);
+    const BankDesc *db = upload(b_sbanks, sb);
+    const int *dbb = upload(b_sblk_bank, blk_bank);
+    const int *dbf = upload(b_sblk_first, blk_first);
+    Hit *dc = b_scands.get<Hit>(cand);
+    launch_bank(s_, A_, db, dbb, dbf, (int)blk_bank.size(), seeds_dev, b_masks.get<uint8_t>(1),
+                b_list_pq.get<P8>(1), b_list_na.get<int>(1), list_cap_, dc, fast_);
+    launched();
+    float *sc = b_pred.get<float>(cand);
+    launch_hit_scores(s_, dc, cand, 1, sc);
+    int *items = b_sitems.get<int>(sb.size());
+    launch_bank_seed_count(s_, db, (int)sb.size(), b_list_na.get<int>(1), items);
+    std::vector<long long> sel_off(sb.size());
+    for (size_t b = 0; b < sb.size(); b++) sel_off[b] = (long long)b * screen;
+    int *isel = b_isel.get<int>(sb.size() * (size_t)screen);
+    launch_select_top(s_, sc, upload(b_srow, row0), items, (int)sb.size(), screen, isel,
+                      upload(b_ssel, sel_off), b_scnt.get<int>(sb.size()),
+                      b_pmax.get<float>(sb.size()));
+    launched();
+    IA_CHECK(hipGetLastError());
+    A_.isel = isel;
+    for (size_t b = 0; b < banks.size(); b++) {
+        banks[b].sel_m = screen;
+        banks[b].sel_off = sel_off[b];
+        banks[b].cand_cap = std::min(banks[b].cand_cap, screen * banks[b].C);
+    }
+}
+
 Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc> &seeds,
-                       size_t n_out, Hit *out_dev) {
+                       size_t n_out, Hit *out_dev, int screen) {
+    if (screen > 0 && !banks.empty()) {
+        // the list copies screen_seeds reads are gathered below as well
+        int max_slot = -1;
+        for (const BankDesc &b : banks)
+            if (b.pts == PTS_LIST) max_slot = std::max(max_slot, b.a0);
+        if (max_slot >= 0) {
+            P8 *lpq = b_list_pq.get<P8>((size_t)(max_slot + 1) * list_cap_);
+            launch_list_gather(s_, A_, upload(b_banks, banks), (int)banks.size(),
+                               b_lists_p.get<int>(1), b_lists_n.get<int>(1),
+                               b_list_na.get<int>(1), list_cap_, lpq);
+            launched();
+        }
+        screen_seeds(banks, upload(b_seeds, seeds), screen);
+    }
     std::vector<int> blk_bank, blk_first;
     long long cand = 0;
     for (size_t b = 0; b < banks.size(); b++) {
@@ -433,7 +505,7 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
     for (const BankDesc &b : banks)
         if (b.pts == PTS_LIST) max_slot = std::max(max_slot, b.a0);
     P8 *lpq = b_list_pq.get<P8>((size_t)(max_slot + 1) * list_cap_);
-    if (max_slot >= 0) {
+    if (max_slot >= 0 && screen <= 0) {
         launch_list_gather(s_, A_, db, (int)banks.size(), b_lists_p.get<int>(1),
                            b_lists_n.get<int>(1), b_list_na.get<int>(1), list_cap_, lpq);
         launched();
@@ -539,7 +611,7 @@ std::vector<Hit> Engine::group_reduce(const std::vector<long long> &offs,
 }
 
 std::vector<Hit> Engine::dp_refine(const std::vector<int> &pairs, const std::vector<Hit> &start,
-                                   int iters, bool lite) {
+                                   int iters, bool lite, int screen) {
     const size_t n = pairs.size();
     std::vector<Hit> res(n, identity_hit(0.0));  // align.h:61-64
     std::vector<int> run;                        // entries with plen, nlen >= 3
@@ -618,7 +690,7 @@ std::vector<Hit> Engine::dp_refine(const std::vector<int> &pairs, const std::vec
             banks.push_back(list_bank(pairs[run[ch / 2]], (int)ch,
                                       kmax[std::min(h.plen, h.nlen)] * 6, h.info.sd8, (long long)ch));
         }
-        Hit *ddet = run_banks(banks, {}, nc);
+        Hit *ddet = run_banks(banks, {}, nc, nullptr, screen);
         const std::vector<Hit> det = download(ddet, nc);
         const std::vector<ListOut> lo = download(b_lout.get<ListOut>(nc), nc);
         for (size_t ch = 0; ch < nc; ch++) {
@@ -774,7 +846,7 @@ void Engine::stage_ss() {
                 const HostPair &h = hp_[E[e]];
                 banks.push_back(list_bank(E[e], (int)e, kmax[std::min(h.plen, h.nlen)] * 6, h.info.sd8, (long long)e));
             }
-            Hit *dd = run_banks(banks, {}, E.size());
+            Hit *dd = run_banks(banks, {}, E.size(), nullptr, learned_ ? top_seeds : 0);
             const std::vector<Hit> det = download(dd, E.size());
             const std::vector<ListOut> lo = download(b_lout.get<ListOut>(E.size()), E.size());
             std::vector<int> dpp;
@@ -789,7 +861,8 @@ void Engine::stage_ss() {
                 }
             }
             if (!dpp.empty()) {
-                const std::vector<Hit> dp = dp_refine(dpp, dps, dp_iter, false);
+                const std::vector<Hit> dp =
+                    dp_refine(dpp, dps, dp_iter, false, learned_ ? top_seeds : 0);
                 for (size_t k = 0; k < dpp.size(); k++)
                     if (dp[k].score > ssb[dpp[k]].score) ssb[dpp[k]] = dp[k];
             }
@@ -1067,7 +1140,7 @@ void Engine::stage_rescue() {
                 launch_fill_none(s_, dthr, base);
                 launched();
             }
-            run_banks(banks, {}, base, dthr);
+            run_banks(banks, {}, base, dthr, learned_ ? top_seeds : 0);
             if (prune) {
                 const long long rows = base / 3;
                 float *sc = b_pred.get<float>(rows);
@@ -1334,6 +1407,7 @@ std::vector<float> g_ranker;
 int g_top_m = 64;
 double g_margin = -1.0;
 int g_top_offsets = 16;
+int g_top_seeds = 0;
 unsigned g_ranker_ver = 0;
 unsigned g_engine_ranker_ver = 0;
 
@@ -1350,6 +1424,7 @@ void sync_ranker() {
     g_engine->top_m = g_top_m;
     g_engine->fallback_margin = g_margin;
     g_engine->top_offsets = g_top_offsets;
+    g_engine->top_seeds = g_top_seeds;
     g_engine_ranker_ver = g_ranker_ver;
 }
 
@@ -1497,14 +1572,21 @@ int indialign_last_stage_scores(double *out, int n_pairs) {
     return m;
 }
 
-int indialign_set_ranker(const float *params, int top_m, double fallback_margin,
-                         int top_offsets) {
-    if (params && (top_m < 1 || top_offsets < 1)) return -1;
+int indialign_set_ranker(const float *params) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (params) g_ranker.assign(params, params + ia::kNParam);
     else g_ranker.clear();
-    g_top_m = params ? top_m : 64;
-    g_top_offsets = params ? top_offsets : 16;
+    g_ranker_ver++;
+    return 0;
+}
+
+int indialign_set_learned_budgets(int top_m, int top_offsets, int top_seeds,
+                                  double fallback_margin) {
+    if (top_m < 1 || top_offsets < 1 || top_seeds < 0) return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_top_m = top_m;
+    g_top_offsets = top_offsets;
+    g_top_seeds = top_seeds;
     g_margin = fallback_margin;
     g_ranker_ver++;
     return 0;

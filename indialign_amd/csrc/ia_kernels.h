@@ -31,6 +31,8 @@ struct BankDesc {
     double et;           // early termination threshold (search.h:83); huge = never
     double sd8;          // d8 cutoff when use_d8
     double cut[kMaxCut];
+    int sel_m;           // > 0: only seeds A.isel[sel_off + i], i < min(seeds, sel_m)
+    long long sel_off;   //      (a screened subset in increasing seed order)
 };
 
 // ---- NW chains (one wavefront per chain) ---------------------------------
@@ -93,6 +95,10 @@ void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int
 void launch_list_gather(hipStream_t s, const Arena &A, const BankDesc *banks, int n_banks,
                         const int *lists_p, const int *lists_n, const int *list_na, int list_cap,
                         P8 *list_pq);
+// out[b] = number of seeds of bank b (implicit seeds depend on the list
+// length, known on the device).
+void launch_bank_seed_count(hipStream_t s, const BankDesc *banks, int n_banks, const int *list_na,
+                            int *out);
 void launch_bank_reduce(hipStream_t s, const BankDesc *banks, int n_banks, const int *list_na,
                         const Hit *cands, Hit *out);
 void launch_group_reduce(hipStream_t s, const long long *offs, const int *counts, int n,

@@ -47,6 +47,12 @@ IA_D int bank_seeds(const BankDesc &B, int n) {
     return B.seeds == SEED_EXPLICIT ? B.K : implicit_seed_count(n, B.max_starts);
 }
 
+// Seeds that run: all, or the screened subset.
+IA_D int bank_live_seeds(const BankDesc &B, int n) {
+    const int k = bank_seeds(B, n);
+    return B.sel_m > 0 && B.sel_m < k ? B.sel_m : k;
+}
+
 // One candidate: iterative_seed_refine (search.h:28-56) or the gapless
 // offset start + refine_from_rt (threading.h:15-37, 415-430), then the
 // TM-score of search.h:358-362.
@@ -56,7 +62,8 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
     const PairInfo &pi = A.info[B.pair];
     const int off = pi.off;
     const int C = B.C;
-    const int k = j / C, c = j - k * C;
+    const int kk = j / C, c = j - kk * C;
+    const int k = B.sel_m > 0 ? A.isel[B.sel_off + kk] : kk;
     SeedDesc sd;
     if (B.seeds == SEED_EXPLICIT) {
         sd = seeds[B.seed_off + k];
@@ -201,7 +208,7 @@ __global__ __launch_bounds__(256) void bank_kernel(Arena A, const BankDesc *bank
     if (j >= B.cand_cap) return;
     Hit h;
     const int n = bank_points(B, list_na);
-    const int K = bank_seeds(B, n);
+    const int K = bank_live_seeds(B, n);
     if (j >= K * B.C || n <= 0) {
         h.score = -INFINITY;
         set_identity(h.R, h.t);
@@ -235,6 +242,14 @@ __global__ __launch_bounds__(256) void list_gather_kernel(Arena A, const BankDes
     }
 }
 
+__global__ void bank_seed_count_kernel(const BankDesc *banks, int n_banks, const int *list_na,
+                                       int *out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_banks) return;
+    const int n = bank_points(banks[b], list_na);
+    out[b] = n > 0 ? bank_seeds(banks[b], n) : 0;
+}
+
 // (score, index) first-wins maximum over a block.
 IA_D void block_argmax(double &s, long long &idx) {
     __shared__ double ss[256];
@@ -264,7 +279,7 @@ __global__ __launch_bounds__(256) void bank_reduce_kernel(const BankDesc *banks,
                                                           Hit *out) {
     const BankDesc &B = banks[blockIdx.x];
     const int n = bank_points(B, list_na);
-    const int total = n > 0 ? bank_seeds(B, n) * B.C : 0;
+    const int total = n > 0 ? bank_live_seeds(B, n) * B.C : 0;
     const Hit *cb = cands + B.cand_off;
     // serial early termination: stop after the first seed whose candidates
     // lift the running best to >= et (search.h:83)
@@ -448,6 +463,12 @@ void launch_list_gather(hipStream_t s, const Arena &A, const BankDesc *banks, in
     if (n_banks <= 0) return;
     list_gather_kernel<<<n_banks, 256, 0, s>>>(A, banks, lists_p, lists_n, list_na, list_cap,
                                                list_pq);
+}
+
+void launch_bank_seed_count(hipStream_t s, const BankDesc *banks, int n_banks, const int *list_na,
+                            int *out) {
+    if (n_banks <= 0) return;
+    bank_seed_count_kernel<<<(n_banks + 255) / 256, 256, 0, s>>>(banks, n_banks, list_na, out);
 }
 
 void launch_bank_reduce(hipStream_t s, const BankDesc *banks, int n_banks, const int *list_na,

@@ -105,12 +105,28 @@ def test_learned_unlimited_budget_is_fast_method():
     _gpu_or_skip()
     pairs = synth.short_pairs(24, first_seed=501)
     sf, Rf, tf = ia.tmscore_search_batch(pairs, serial_bank=1, method=ia.METHOD_FAST)
-    ia.set_ranker(ia.init_params(0), top_m=1 << 30, top_offsets=1 << 30)
+    ia.set_ranker(ia.init_params(0), top_m=1 << 30, top_offsets=1 << 30, top_seeds=0)
     sl, Rl, tl = ia.tmscore_search_batch(pairs, serial_bank=1, method=ia.METHOD_LEARNED)
     ia.set_ranker(None)
     np.testing.assert_array_equal(sl, sf)
     np.testing.assert_array_equal(Rl, Rf)
     np.testing.assert_array_equal(tl, tf)
+
+
+def test_seed_screening_with_full_budget_changes_nothing():
+    """Seed screening keeping at least every seed of every bank selects them
+    all in order: the same results as no screening, bit for bit."""
+    _gpu_or_skip()
+    pairs = synth.short_pairs(16, first_seed=801)
+    p = ia.init_params(0)
+    ia.set_ranker(p, top_m=64, top_offsets=16, top_seeds=0)
+    s0, R0, t0 = ia.tmscore_search_batch(pairs, serial_bank=1, method=ia.METHOD_LEARNED)
+    ia.set_ranker(p, top_m=64, top_offsets=16, top_seeds=1 << 20)
+    s1, R1, t1 = ia.tmscore_search_batch(pairs, serial_bank=1, method=ia.METHOD_LEARNED)
+    ia.set_ranker(None)
+    np.testing.assert_array_equal(s1, s0)
+    np.testing.assert_array_equal(R1, R0)
+    np.testing.assert_array_equal(t1, t0)
 
 
 def test_learned_without_ranker_is_an_error():
