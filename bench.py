@@ -22,7 +22,11 @@ packaged trained ranker are refined), the `fast` object method 1 (one step);
 both give their TM loss against the plain FP64 scores of the same pairs,
 overall and per pipeline stage.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs P]
+`--workload long` runs the long-pair configuration instead (512 pairs, N 400-1500,
+masks and domain motion; use --steps 1 --warmup 0 --cpu-sample 2: one plain search
+takes minutes).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--pairs P] [--workload short|long]
 """
 from __future__ import annotations
 
@@ -61,8 +65,17 @@ def cpu_baseline(pairs, sample):
     dt = time.perf_counter() - t0
     return {"value": round(len(sub) / dt, 4), "unit": "pairs/s",
             "cores": int(os.environ.get("OMP_NUM_THREADS", cores)), "kind": "reference",
-            "sample": f"first {len(sub)} pairs of the same workload (seeds 1..{len(sub)}), "
-                      f"reference batch entry, {dt:.1f} s"}
+            "sample": f"first {len(sub)} pairs of the same workload, reference batch entry, "
+                      f"{dt:.1f} s"}
+
+
+def workload_name(kind, n):
+    if kind == "short":
+        return (f"{n} pairs, seeds 1..{n}, N~U{{40,300}}, sigma~U(0.5,14) A, plain FP64 method, "
+                "batch semantics")
+    return (f"{n} pairs, seeds 20001..{20000 + n}, N~U{{400,1500}}, sigma~U(2,10) A, 20% of helix "
+            "segments moved as rigid domains, 10% of pred and native residues masked, Lnorm = "
+            "native valid count, plain FP64 method, batch semantics")
 
 
 def stage_delta(ia, plain_stage, stage):
@@ -129,7 +142,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--pairs", type=int, default=4096)
+    ap.add_argument("--pairs", type=int, default=None,
+                    help="pairs in the batch (default 4096 short, 512 long)")
+    ap.add_argument("--workload", choices=["short", "long"], default="short",
+                    help="short: N~U{40,300} (the headline); long: N~U{400,1500} with masks "
+                         "and domain motion")
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
@@ -155,8 +172,10 @@ def main():
     import indialign_amd as ia
     from indialign_amd import synth
 
+    if args.pairs is None:
+        args.pairs = 4096 if args.workload == "short" else 512
     t0 = time.perf_counter()
-    pairs = synth.short_pairs(args.pairs)
+    pairs = synth.short_pairs(args.pairs) if args.workload == "short" else synth.long_pairs(args.pairs)
     log(f"[bench] generated {len(pairs)} pairs in {time.perf_counter() - t0:.1f} s")
     if not ia.gpu_available():
         sys.exit("bench.py: no GPU visible")
@@ -195,8 +214,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{len(pairs)} pairs, seeds 1..{len(pairs)}, N~U{{40,300}}, "
-                               "sigma~U(0.5,14) A, plain FP64 method, batch semantics",
+        "config": {"workload": workload_name(args.workload, len(pairs)),
                    "pairs": len(pairs), "method": "plain-fp64"},
         "tm_loss_vs_plain": {"mean": 0.0, "max": 0.0},
         "candidate_evals_per_s": round(st["candidates"] / (st["wall_ms"] / 1e3), 1),
