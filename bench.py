@@ -129,7 +129,7 @@ def method_leg(ia, cfg, plain_scores, plain_stage, args, plain_s_per_step, metho
 
 def measured_traffic():
     """HBM bytes per chain-kernel launch from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (profiles/chain_kernel_short512.md)."""
+    FETCH_SIZE / WRITE_SIZE summary (profiles/chain_kernel_short4096.md)."""
     p = os.path.join(ROOT, "profiles", "chain_kernel_traffic.json")
     if not os.path.exists(p):
         return None
@@ -149,7 +149,9 @@ def main():
                          "and domain motion")
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--legs", default="learned,fast",
+                    help="comma list of the methods timed after the plain one: learned, fast "
+                         "(empty: plain only)")
     ap.add_argument("--top-m", type=int, default=64)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy kernel of this size (counter calibration)")
@@ -225,9 +227,11 @@ def main():
                      "traffic": measured_traffic()},
         "mean_score": round(float(scores.mean()), 6),
     }
-    if not args.no_learned:
+    legs = [x for x in args.legs.split(",") if x]
+    if "learned" in legs:
         out["learned"] = method_leg(ia, cfg, scores, plain_stage, args, elapsed / args.steps,
                                     ia.METHOD_LEARNED, args.steps, args.warmup)
+    if "fast" in legs:
         out["fast"] = method_leg(ia, cfg, scores, plain_stage, args, elapsed / args.steps,
                                  ia.METHOD_FAST, 1, 0)
     if not args.no_cpu:
