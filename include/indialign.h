@@ -149,9 +149,10 @@ int indialign_rotation_from_covariance(const double *h, int method, double *R);
  * method 2.  Returns 0. */
 int indialign_set_ranker(const float *params);
 
-/* Work budgets of method 2 (defaults 64, 16, 0, -1):
- *   top_m >= 1: local-fragment work items refined per pair and pass, the
- *     ranker's top ones;
+/* Work budgets of method 2 (defaults 64, 16, 16, 0, -1):
+ *   top_m >= 1: dense local-fragment work items refined per pair (rescue.h
+ *     dense pass), the ranker's top ones;
+ *   top_m_first >= 1: the same for the first local-fragment pass;
  *   top_offsets >= 1: threading-DP offsets (rescue.h:121-190) per pair that
  *     start NW chains, chosen by their seed-bank score (every offset still
  *     runs its seed bank);
@@ -164,7 +165,7 @@ int indialign_set_ranker(const float *params);
  *     its running best + fallback_margin refines every local-fragment work
  *     item of that pass, as the plain search does (< 0: never).
  * Returns 0, or -1 for a budget out of range. */
-int indialign_set_learned_budgets(int top_m, int top_offsets, int top_seeds,
+int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets, int top_seeds,
                                   double fallback_margin);
 
 /* Plain search (method 0, serial seed bank) that also records every

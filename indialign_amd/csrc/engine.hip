@@ -130,7 +130,8 @@ class Engine {
     std::vector<double> stage_scores;
     // Work-item ranker used by method 2 and its per-pair budget.
     Mlp ranker;
-    int top_m = 64;
+    int top_m = 64;                 // dense local-fragment pass
+    int top_m_first = 16;           // first local-fragment pass
     double fallback_margin = -1.0;  // < 0: never fall back to every work item
     int top_offsets = 16;           // threading-DP offsets per pair that start chains
     int top_seeds = 0;              // seeds refined per SS-stage and threading bank (0 = all)
@@ -996,6 +997,7 @@ void Engine::stage_rescue() {
         }
         if (rank) {
             // refine only the ranker's top-m work items of every pair
+            const int top_m = dense ? this->top_m : top_m_first;
             float *pred = b_pred.get<float>(rows);
             ranker.forward(s_, X, rows, pred);
             std::vector<long long> row0(fgs.size()), sel_off(fgs.size());
@@ -1405,6 +1407,7 @@ double g_stats[8];
 // engine re-created after a device error gets them again).
 std::vector<float> g_ranker;
 int g_top_m = 64;
+int g_top_m_first = 16;
 double g_margin = -1.0;
 int g_top_offsets = 16;
 int g_top_seeds = 0;
@@ -1422,6 +1425,7 @@ void sync_ranker() {
     if (g_ranker.empty()) g_engine->ranker.unload();
     else g_engine->ranker.set_params(g_engine->stream(), g_ranker.data());
     g_engine->top_m = g_top_m;
+    g_engine->top_m_first = g_top_m_first;
     g_engine->fallback_margin = g_margin;
     g_engine->top_offsets = g_top_offsets;
     g_engine->top_seeds = g_top_seeds;
@@ -1580,11 +1584,12 @@ int indialign_set_ranker(const float *params) {
     return 0;
 }
 
-int indialign_set_learned_budgets(int top_m, int top_offsets, int top_seeds,
+int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets, int top_seeds,
                                   double fallback_margin) {
-    if (top_m < 1 || top_offsets < 1 || top_seeds < 0) return -1;
+    if (top_m < 1 || top_m_first < 1 || top_offsets < 1 || top_seeds < 0) return -1;
     std::lock_guard<std::mutex> lk(g_mu);
     g_top_m = top_m;
+    g_top_m_first = top_m_first;
     g_top_offsets = top_offsets;
     g_top_seeds = top_seeds;
     g_margin = fallback_margin;
