@@ -336,7 +336,20 @@ struct KabschSeq {
 };
 
 // TM term 1/(1 + d2/d0sq) (core.h:147,163).
-IA_HD double tm_term(double d2, double d0sq) { return 1.0 / (1.0 + d2 / d0sq); }
+// a / b rounded to nearest from inv = RN(1 / b): a faithful quotient a * inv
+// corrected once with an exact FMA remainder is the correctly rounded
+// quotient (Markstein's theorem; no underflow or overflow in this range).
+// Bit-identical to a / b, at a multiply and two FMAs when b is loop-invariant.
+IA_HD double div_by(double a, double b, double inv) {
+    const double q = a * inv;
+    return fma(fma(-b, q, a), inv, q);
+}
+
+// TM term 1/(1 + d2/d0sq) (core.h:147,163), rounded exactly as written; the
+// reciprocal of d0sq is loop-invariant wherever this runs in a loop.
+IA_HD double tm_term(double d2, double d0sq) {
+    return 1.0 / (1.0 + div_by(d2, d0sq, 1.0 / d0sq));
+}
 
 // ---- window seeds over n points (seeds.h:15-37), computed on the fly ----
 // Returns the number of distinct window lengths (descending) in lens.

@@ -181,7 +181,7 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
                         tm = __builtin_amdgcn_rcpf(__builtin_fmaf(d2, inv_d0sq, 1.0f));
                     } else {
                         d2 = dx * dx + dy * dy + dz * dz;
-                        tm = 1.0 / (1.0 + d2 / d0sq_s);
+                        tm = 1.0 / (1.0 + div_by(d2, d0sq_s, inv_d0sq));
                     }
                     if (SMODE == SM_TM) {
                         s = (d2 <= cut8) ? tm : (V)0;

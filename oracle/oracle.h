@@ -57,6 +57,11 @@ int oracle_search_traced(const OraclePair *p, const OracleConfig *cfg, OracleRes
 int oracle_search_batch(int n, const OraclePair *p, const OracleConfig *cfg,
                         OracleResult *out, int serial_bank);
 
+/* Mismatches of the FMA-corrected quotient (a * RN(1/b), one correction)
+ * against a / b over n random cases (the identity the GPU kernels use for
+ * d2 / d0^2). */
+long long oracle_markstein_mismatches(long long n, unsigned long long seed);
+
 /* Single functions (inputs AoS xyz like the reference). */
 void oracle_d0(double L, double *out3);
 void oracle_jacobi4(double *A16, double *V16);

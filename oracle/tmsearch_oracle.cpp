@@ -16,6 +16,7 @@
 #include <unordered_map>
 #include <unordered_set>
 #include <utility>
+#include <random>
 #include <vector>
 
 namespace {
@@ -986,6 +987,49 @@ int oracle_search_batch(int n, const OraclePair *p, const OracleConfig *cfg, Ora
 }
 
 void oracle_d0(double L, double *o) { d0_params(L, o[0], o[1], o[2]); }
+
+// Correctly rounded division by a loop-invariant divisor from its correctly
+// rounded reciprocal and one FMA correction (Markstein), the form the GPU
+// kernels use for d2 / d0^2.  Counts the cases, among n random ones, where it
+// differs from a / b; the draws cover d0^2 of the TM-score formula,
+// arbitrary mantissas over 2^-4..2^11 and quotients over 2^-60..2^14.
+long long oracle_markstein_mismatches(long long n, unsigned long long seed) {
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    long long bad = 0;
+    for (long long k = 0; k < n; k++) {
+        double a, b;
+        switch (k % 4) {
+            case 0: {
+                double d0, d0s, sd8;
+                d0_params(3.0 + U(rng) * 3000.0, d0, d0s, sd8);
+                b = d0 * d0;
+                a = U(rng) * 1e4;
+                break;
+            }
+            case 1: {
+                uint64_t u = (rng() & 0x000FFFFFFFFFFFFFull) | (0x3FFull << 52), v;
+                std::memcpy(&b, &u, 8);
+                v = (rng() & 0x000FFFFFFFFFFFFFull) | ((0x3F0ull + rng() % 40) << 52);
+                std::memcpy(&a, &v, 8);
+                b = std::ldexp(b, (int)(rng() % 16) - 4);
+                break;
+            }
+            case 2:
+                b = std::ldexp(1.0 + std::ldexp((double)(rng() % 1024), -52), (int)(rng() % 10) - 3);
+                a = U(rng) * 1e3;
+                break;
+            default:
+                b = 0.25 + U(rng) * 400.0;
+                a = std::ldexp(U(rng), -(int)(rng() % 60));
+        }
+        const double inv = 1.0 / b;
+        const double q0 = a * inv;
+        const double q = std::fma(std::fma(-b, q0, a), inv, q0);
+        bad += q != a / b;
+    }
+    return bad;
+}
 
 void oracle_jacobi4(double *A16, double *V16) {
     double a[4][4], v[4][4];
