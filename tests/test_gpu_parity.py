@@ -90,6 +90,21 @@ def test_long_partial_pairs_vs_oracle():
     _check(s, R, t, ref, "long")
 
 
+def test_long_pairs_global_trace_vs_oracle():
+    """The long-pair configuration (N 400-1500, domain motion, masks): six
+    pairs of N 541-1122, whose NW traces exceed LDS and live in per-wavefront
+    global slots, and one noisy N = 600 pair that runs every rescue pass,
+    dense included, at that size."""
+    _gpu_or_skip()
+    pairs = synth.long_pairs(6) + [synth.make_pair(20999, n_range=(600, 600),
+                                                   sigma_range=(12.0, 12.0), domain_motion=True,
+                                                   drop_frac=0.10)]
+    s, R, t = ia.tmscore_search_batch(pairs)
+    ref = libs.oracle_search(pairs, serial_bank=1)
+    _check(s, R, t, ref, "long-global-trace")
+    assert ref[-1, 12] < 0.3  # the dense rescue pass ran
+
+
 def test_reference_toy_pair():
     """The 5-residue toy of the reference tests (tests/test_gpu_scoring_refactor.py:179-190):
     the reference C path scores it 0.77317584653223026 for dp_iter 0 and 1."""
