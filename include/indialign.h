@@ -149,13 +149,15 @@ int indialign_rotation_from_covariance(const double *h, int method, double *R);
  * method 2.  Returns 0. */
 int indialign_set_ranker(const float *params);
 
-/* Work budgets of method 2 (defaults 64, 16, 16, 0, -1):
+/* Work budgets of method 2 (defaults 64, 16, 16, 32, 0, -1):
  *   top_m >= 1: dense local-fragment work items refined per pair (rescue.h
  *     dense pass), the ranker's top ones;
  *   top_m_first >= 1: the same for the first local-fragment pass;
  *   top_offsets >= 1: threading-DP offsets (rescue.h:121-190) per pair that
- *     start NW chains, chosen by their seed-bank score (every offset still
- *     runs its seed bank);
+ *     start NW chains, chosen by their seed-bank score;
+ *   top_offset_banks: threading-DP offsets per pair that run their seed
+ *     bank, chosen by the TM-score of their window's own superposition
+ *     (0: every offset; else at least top_offsets);
  *   top_seeds >= 0: seeds refined per bank of the SS-seed stage (its
  *     detailed searches and DP refinement, ss.h:56-164) and of threading DP,
  *     chosen by the TM-score of each seed's initial superposition; 0 refines
@@ -165,8 +167,8 @@ int indialign_set_ranker(const float *params);
  *     its running best + fallback_margin refines every local-fragment work
  *     item of that pass, as the plain search does (< 0: never).
  * Returns 0, or -1 for a budget out of range. */
-int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets, int top_seeds,
-                                  double fallback_margin);
+int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets,
+                                  int top_offset_banks, int top_seeds, double fallback_margin);
 
 /* Plain search (method 0, serial seed bank) that also records every
  * local-fragment work item: its descriptor row (X, INDIALIGN_MLP_NFEAT floats

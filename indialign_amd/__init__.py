@@ -91,7 +91,7 @@ def load_library(path: str | None = None):
     lib.indialign_set_ranker.argtypes = [c_float_p]
     lib.indialign_set_ranker.restype = ctypes.c_int
     lib.indialign_set_learned_budgets.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                                  ctypes.c_int, ctypes.c_double]
+                                                  ctypes.c_int, ctypes.c_int, ctypes.c_double]
     lib.indialign_set_learned_budgets.restype = ctypes.c_int
     lib.indialign_collect_records.argtypes = [
         ctypes.c_int, ctypes.POINTER(NativePairInput), ctypes.POINTER(NativeConfig),
@@ -259,13 +259,15 @@ def _f32(a) -> np.ndarray:
 
 
 def set_ranker(params, top_m: int = 64, fallback_margin: float = -1.0,
-               top_offsets: int = 16, top_seeds: int = 0, top_m_first: int = 16) -> None:
+               top_offsets: int = 16, top_seeds: int = 0, top_m_first: int = 16,
+               top_offset_banks: int = 32) -> None:
     """Install the ranker (NPARAM float32 values) used by METHOD_LEARNED, or
     remove it with params=None, and set the method's budgets
     (indialign_set_learned_budgets): top_m dense and top_m_first first-pass
     local-fragment work items per pair, top_offsets threading-DP offsets per
-    pair that start NW chains, top_seeds seeds refined per SS-stage and
-    threading bank (0 = all), and the fallback margin (< 0: off)."""
+    pair that start NW chains, top_offset_banks threading-DP offsets per pair
+    that run their seed bank (0 = all), top_seeds seeds refined per SS-stage
+    and threading bank (0 = all), and the fallback margin (< 0: off)."""
     lib = load_library()
     if params is None:
         lib.indialign_set_ranker(None)
@@ -274,7 +276,8 @@ def set_ranker(params, top_m: int = 64, fallback_margin: float = -1.0,
     if p.size != NPARAM:
         raise ValueError(f"ranker needs {NPARAM} parameters, got {p.size}")
     rc = lib.indialign_set_learned_budgets(int(top_m), int(top_m_first), int(top_offsets),
-                                           int(top_seeds), float(fallback_margin))
+                                           int(top_offset_banks), int(top_seeds),
+                                           float(fallback_margin))
     if rc != 0:
         raise IndialignError(f"indialign_set_learned_budgets returned {rc}")
     rc = lib.indialign_set_ranker(p.ctypes.data_as(_f32p))
@@ -363,8 +366,8 @@ DEFAULT_RANKER = os.path.join(HERE, "ranker_params.npy")
 
 def load_default_ranker(top_m: int = 64, fallback_margin: float = -1.0,
                         top_offsets: int = 16, top_seeds: int = 0, top_m_first: int = 16,
-                        path: str | None = None) -> np.ndarray:
+                        top_offset_banks: int = 32, path: str | None = None) -> np.ndarray:
     """Install the packaged trained ranker (train_ranker.py) for METHOD_LEARNED."""
     p = np.load(path or DEFAULT_RANKER, allow_pickle=False)
-    set_ranker(p, top_m, fallback_margin, top_offsets, top_seeds, top_m_first)
+    set_ranker(p, top_m, fallback_margin, top_offsets, top_seeds, top_m_first, top_offset_banks)
     return p

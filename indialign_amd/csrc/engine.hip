@@ -134,6 +134,7 @@ class Engine {
     int top_m_first = 16;           // first local-fragment pass
     double fallback_margin = -1.0;  // < 0: never fall back to every work item
     int top_offsets = 16;           // threading-DP offsets per pair that start chains
+    int top_offset_banks = 32;      // threading-DP offsets per pair that run their bank
     int top_seeds = 0;              // seeds refined per SS-stage and threading bank (0 = all)
     // Record sink: when set, every local-fragment work item of the search
     // appends its descriptor row, its refined score (best of its two
@@ -197,7 +198,7 @@ class Engine {
     DBuf b_px, b_py, b_pz, b_qx, b_qy, b_qz, b_v, b_pv, b_nv, b_pcx, b_pcy, b_pcz, b_qcx, b_qcy,
         b_qcz, b_pmap, b_nmap, b_ssp, b_ssn, b_info, b_qfx, b_qfy, b_qfz, b_p4, b_q4, b_pc4,
         b_qc4, b_ss_scr, b_fblk_g, b_fblk_f, b_pq8, b_list_pq, b_sbanks, b_sblk_bank,
-        b_sblk_first, b_scands;
+        b_sblk_first, b_scands, b_oitems;
     // work buffers
     DBuf b_seeds, b_masks, b_banks, b_blk_bank, b_blk_first, b_cands, b_bank_out, b_lists_p,
         b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
@@ -209,7 +210,7 @@ class Engine {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
                 &b_pcz, &b_qcx, &b_qcy, &b_qcz, &b_pmap, &b_nmap, &b_ssp, &b_ssn, &b_info,
                 &b_qfx, &b_qfy, &b_qfz, &b_p4, &b_q4, &b_pc4, &b_qc4, &b_ss_scr, &b_fblk_g, &b_fblk_f, &b_pq8, &b_list_pq, &b_sbanks, &b_sblk_bank, &b_sblk_first,
-                &b_scands,
+                &b_scands, &b_oitems,
                 &b_seeds, &b_masks, &b_banks, &b_blk_bank, &b_blk_first, &b_cands,
                 &b_bank_out, &b_lists_p, &b_lists_n, &b_list_na, &b_lout, &b_groups, &b_xf,
                 &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
@@ -1077,10 +1078,11 @@ void Engine::stage_rescue() {
         for (int a : live_)
             if (best_[a].score < rs1) P.push_back(a);
         if (!P.empty()) {
-            std::vector<BankDesc> banks;
             std::vector<ChainGroup> groups;
             std::vector<long long> offs;
             std::vector<int> cnts;
+            std::vector<OffsetItem> items;  // every offset window, pair by pair
+            std::vector<long long> slot;    // its result slot (3 per offset)
             long long base = 0;
             static const double mult[6] = {2.0, 1.5, 1.0, 0.75, 0.5, 0.25};
             for (int a : P) {
@@ -1093,24 +1095,8 @@ void Engine::stage_rescue() {
                         const int ps = std::max(off, 0), ns = std::max(-off, 0);
                         const int ov = std::min(pl - ps, nl - ns);
                         if (ov < 3) continue;
-                        BankDesc b{};
-                        b.pair = a;
-                        b.pts = PTS_CWIN;
-                        b.n = ov;
-                        b.a0 = ps;
-                        b.a1 = ns;
-                        b.seeds = SEED_IMPLICIT;
-                        b.max_starts = 5;  // rescue.h:159-161
-                        b.C = 6;
-                        for (int c = 0; c < 6; c++) b.cut[c] = h.info.d0s * mult[c];
-                        b.iters = 8;
-                        b.use_d8 = 1;
-                        b.sd8 = 100.0;
-                        b.init = INIT_SEED;
-                        b.et = 1.0;
-                        b.cand_cap = implicit_seed_count(ov, 5) * 6;
-                        b.out_idx = base + 3LL * no;
-                        banks.push_back(b);
+                        items.push_back(OffsetItem{a, ps, ns, ov});
+                        slot.push_back(base + 3LL * no);
                         no++;
                     }
                 }
@@ -1138,6 +1124,56 @@ void Engine::stage_rescue() {
             // their bank score start NW chains; the other chain slots keep
             // "no result"
             const bool prune = learned_ && top_offsets > 0 && !groups.empty();
+            std::vector<char> keep(items.size(), 1);
+            if (prune && top_offset_banks > 0) {
+                // and only the top_offset_banks offsets of each pair by the
+                // TM-score of their window's own superposition run the bank
+                float *sc = b_pred.get<float>(items.size());
+                launch_offset_screen(s_, A_, upload(b_oitems, items), (int)items.size(), sc);
+                launched();
+                const std::vector<float> score = download(sc, items.size());
+                std::vector<int> ord;
+                size_t b0 = 0;
+                while (b0 < items.size()) {
+                    size_t b1 = b0;
+                    while (b1 < items.size() && items[b1].pair == items[b0].pair) b1++;
+                    if ((int)(b1 - b0) > top_offset_banks) {
+                        ord.resize(b1 - b0);
+                        for (size_t k = 0; k < ord.size(); k++) ord[k] = (int)k;
+                        std::nth_element(ord.begin(), ord.begin() + top_offset_banks, ord.end(),
+                                         [&](int x, int y) {
+                                             const float sx = score[b0 + x], sy = score[b0 + y];
+                                             return sx > sy || (sx == sy && x < y);
+                                         });
+                        for (size_t k = b0; k < b1; k++) keep[k] = 0;
+                        for (int k = 0; k < top_offset_banks; k++) keep[b0 + ord[k]] = 1;
+                    }
+                    b0 = b1;
+                }
+            }
+            std::vector<BankDesc> banks;
+            for (size_t k = 0; k < items.size(); k++) {
+                if (!keep[k]) continue;
+                const OffsetItem &it = items[k];
+                BankDesc b{};
+                b.pair = it.pair;
+                b.pts = PTS_CWIN;
+                b.n = it.ov;
+                b.a0 = it.ps;
+                b.a1 = it.ns;
+                b.seeds = SEED_IMPLICIT;
+                b.max_starts = 5;  // rescue.h:159-161
+                b.C = 6;
+                for (int c = 0; c < 6; c++) b.cut[c] = hp_[it.pair].info.d0s * mult[c];
+                b.iters = 8;
+                b.use_d8 = 1;
+                b.sd8 = 100.0;
+                b.init = INIT_SEED;
+                b.et = 1.0;
+                b.cand_cap = implicit_seed_count(it.ov, 5) * 6;
+                b.out_idx = slot[k];
+                banks.push_back(b);
+            }
             if (prune) {
                 launch_fill_none(s_, dthr, base);
                 launched();
@@ -1410,6 +1446,7 @@ int g_top_m = 64;
 int g_top_m_first = 16;
 double g_margin = -1.0;
 int g_top_offsets = 16;
+int g_top_offset_banks = 32;
 int g_top_seeds = 0;
 unsigned g_ranker_ver = 0;
 unsigned g_engine_ranker_ver = 0;
@@ -1428,6 +1465,7 @@ void sync_ranker() {
     g_engine->top_m_first = g_top_m_first;
     g_engine->fallback_margin = g_margin;
     g_engine->top_offsets = g_top_offsets;
+    g_engine->top_offset_banks = g_top_offset_banks;
     g_engine->top_seeds = g_top_seeds;
     g_engine_ranker_ver = g_ranker_ver;
 }
@@ -1584,10 +1622,13 @@ int indialign_set_ranker(const float *params) {
     return 0;
 }
 
-int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets, int top_seeds,
-                                  double fallback_margin) {
-    if (top_m < 1 || top_m_first < 1 || top_offsets < 1 || top_seeds < 0) return -1;
+int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets,
+                                  int top_offset_banks, int top_seeds, double fallback_margin) {
+    if (top_m < 1 || top_m_first < 1 || top_offsets < 1 || top_offset_banks < 0 ||
+        (top_offset_banks > 0 && top_offset_banks < top_offsets) || top_seeds < 0)
+        return -1;
     std::lock_guard<std::mutex> lk(g_mu);
+    g_top_offset_banks = top_offset_banks;
     g_top_m = top_m;
     g_top_m_first = top_m_first;
     g_top_offsets = top_offsets;

@@ -192,6 +192,15 @@ void launch_select_top(hipStream_t s, const float *pred, const long long *row0,
 void launch_fill_none(hipStream_t s, Hit *out, long long n);
 // y[i] = (float) in[i * stride].score for i < n.
 void launch_hit_scores(hipStream_t s, const Hit *in, long long n, int stride, float *y);
+// Threading-DP offset screening: per offset window (compact pred rows
+// [ps, ps+ov) against native rows [ns, ns+ov)), the TM-score (d0 of the
+// pair, normalized by Lnorm) of the window's own superposition; one lane per
+// offset.
+struct OffsetItem {
+    int pair, ps, ns, ov;
+};
+void launch_offset_screen(hipStream_t s, const Arena &A, const OffsetItem *items, int n,
+                          float *score);
 // y[i] = best score of the two chains of work item i (out[2i], out[2i+1]).
 void launch_item_best(hipStream_t s, const Hit *out, long long n_items, float *y);
 // dst[r] = src[idx[r]] (rows of kFeat values), tdst[r] = tsrc[idx[r]]

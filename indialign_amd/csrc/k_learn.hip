@@ -493,6 +493,31 @@ __global__ void gather_rows_kernel(const float *src, const float *tsrc, const in
     if (c == 0) tdst[r] = tsrc[s];
 }
 
+__global__ void offset_screen_kernel(Arena A, const OffsetItem *items, int n, float *score) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const OffsetItem it = items[i];
+    const PairInfo &pi = A.info[it.pair];
+    const int off = pi.off;
+    KabschSeq ks;
+    ks.reset();
+    for (int q = 0; q < it.ov; q++) {
+        const P4 P = A.pc4[off + it.ps + q], Q = A.qc4[off + it.ns + q];
+        ks.add_moment(P.x, P.y, P.z, Q.x, Q.y, Q.z);
+    }
+    ks.finish_moments();
+    double R[9], t[3];
+    ks.template solve<true>(R, t);
+    const double d0sq = pi.d0 * pi.d0;
+    double acc = 0.0;
+    for (int q = 0; q < it.ov; q++) {
+        const P4 P = A.pc4[off + it.ps + q], Q = A.qc4[off + it.ns + q];
+        const double d2 = dist2_moved_fma(R, t, P.x, P.y, P.z, Q.x, Q.y, Q.z);
+        acc += d0sq / (d0sq + d2);
+    }
+    score[i] = (float)(acc / fmax(pi.Lnorm, 1.0));
+}
+
 __global__ void fill_none_kernel(Hit *out, long long n) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -516,6 +541,13 @@ __global__ void item_best_kernel(const Hit *out, long long n, float *y) {
 }
 
 }  // namespace
+
+void launch_offset_screen(hipStream_t s, const Arena &A, const OffsetItem *items, int n,
+                          float *score) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(offset_screen_kernel, dim3((n + 255) / 256), dim3(256), 0, s, A, items, n,
+                       score);
+}
 
 void launch_fill_none(hipStream_t s, Hit *out, long long n) {
     if (n <= 0) return;

@@ -106,7 +106,7 @@ def test_learned_unlimited_budget_is_fast_method():
     pairs = synth.short_pairs(24, first_seed=501)
     sf, Rf, tf = ia.tmscore_search_batch(pairs, serial_bank=1, method=ia.METHOD_FAST)
     ia.set_ranker(ia.init_params(0), top_m=1 << 30, top_offsets=1 << 30, top_seeds=0,
-                  top_m_first=1 << 30)
+                  top_m_first=1 << 30, top_offset_banks=0)
     sl, Rl, tl = ia.tmscore_search_batch(pairs, serial_bank=1, method=ia.METHOD_LEARNED)
     ia.set_ranker(None)
     np.testing.assert_array_equal(sl, sf)
