@@ -551,18 +551,22 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     IA_CHECK(hipMemsetAsync(q, 0, sizeof(int), s_));
     const int wpr = (mn + 15) / 16;
     const size_t trace_bytes = (size_t)mp * wpr * 4;
-    // LDS layout: trace and staged columns in LDS when they fit, else the
-    // trace in a per-wavefront global slot, else the columns from global too.
+    // LDS layout: the trace and the staged columns each go to LDS only when
+    // that costs no resident wavefront (the fill is latency-bound, so
+    // occupancy beats the global slots' extra traffic); otherwise the trace
+    // lives in a per-wavefront global slot and the columns are read from the
+    // arena.
     constexpr size_t kLdsMax = 160 * 1024;
-    const bool in_lds = trace_bytes <= 96 * 1024 && chain_lds_bytes(mp, mn, true, true) <= kLdsMax;
-    // stage the columns only when that costs no resident wavefront
-    const size_t lds_c = chain_lds_bytes(mp, mn, in_lds, true);
-    const size_t lds_g = chain_lds_bytes(mp, mn, in_lds, false);
-    const bool cols_lds = !fast_ && lds_c <= kLdsMax &&
-                          kLdsMax / lds_c >= std::min<size_t>(8, kLdsMax / lds_g);
-    const size_t lds = cols_lds ? lds_c : lds_g;
-    if (lds > kLdsMax) throw DevError("chain too long for one workgroup's LDS");
-    int per_cu = std::max(1, chain_blocks_per_cu(lds));
+    auto occupancy = [&](bool tr, bool cl) {
+        const size_t b = chain_lds_bytes(mp, mn, tr, cl);
+        return b > kLdsMax ? 0 : chain_blocks_per_cu(b, fast_);
+    };
+    const int occ_free = occupancy(false, false);
+    if (occ_free <= 0) throw DevError("chain too long for one workgroup's LDS");
+    const bool in_lds = trace_bytes <= 96 * 1024 && occupancy(true, false) >= occ_free;
+    const bool cols_lds = !fast_ && occupancy(in_lds, true) >= occupancy(in_lds, false);
+    const size_t lds = chain_lds_bytes(mp, mn, in_lds, cols_lds);
+    int per_cu = std::max(1, chain_blocks_per_cu(lds, fast_));
     per_cu = std::min(per_cu, 16);
     long long slots = (long long)n_cu_ * per_cu;
     unsigned *trace = nullptr;

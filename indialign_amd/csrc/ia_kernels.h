@@ -208,6 +208,6 @@ void launch_gather_rows(hipStream_t s, const float *src, const float *tsrc, cons
                         int n, float *dst, float *tdst);
 
 size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds);
-int chain_blocks_per_cu(size_t lds);
+int chain_blocks_per_cu(size_t lds, bool fast);
 
 }  // namespace ia

@@ -10,8 +10,10 @@
 // neighbours arrive from lane l-1 by a cross-lane shift and its left
 // neighbour is the lane's own previous result.  The last row of a strip is
 // handed to the next strip through an LDS row buffer.  Trace codes
-// (0 diag, 1 up, 2 left) are packed 16 cells per 32-bit word into LDS (or a
-// per-wavefront global slot for very long chains) and walked back by lane 0.
+// (0 diag, 1 up, 2 left) are packed 16 cells per 32-bit word into LDS, or
+// into a per-wavefront global slot when the LDS copy would cost resident
+// wavefronts (long chains, or 40-300 nt batches at three wavefronts per
+// SIMD), and walked back from there.
 //
 // Work distribution: a persistent grid pulls chain indices from an atomic
 // queue, so chains of very different cost balance across the CUs.
@@ -247,25 +249,67 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
         nw_fill<SMODE, TR, CL, 2, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
     __builtin_amdgcn_wave_barrier();
-    // traceback (nw.h:42-68) by lane 0
-    if (lane == 0) {
+    // traceback (nw.h:42-68)
+    if constexpr (!std::is_same<TR, unsigned>::value) {
+        // trace in LDS: lane 0 walks it
+        if (lane == 0) {
+            int i = plen, j = nlen, k = 0;
+            while (i > 0 && j > 0) {
+                const unsigned w = trace[(long long)(i - 1) * wpr + ((j - 1) >> 4)];
+                const unsigned d = (w >> (((j - 1) & 15) * 2)) & 3u;
+                if (d == 0) {
+                    L.alP[k] = (short)(i - 1);
+                    L.alN[k] = (short)(j - 1);
+                    k++;
+                    i--;
+                    j--;
+                } else if (d == 1) {
+                    i--;
+                } else {
+                    j--;
+                }
+            }
+            L.misc[0] = k;
+        }
+    } else {
+        // trace in global memory: one round of loads brings a window of 64
+        // rows (lane l holds row top-1-l) by two words (the current word
+        // column c1 and c1-1) into registers; the wavefront walks it with
+        // wave-uniform (i, j) and fetches each step's word by readlane,
+        // reloading when the path leaves the window.
         int i = plen, j = nlen, k = 0;
         while (i > 0 && j > 0) {
-            const unsigned w = trace[(long long)(i - 1) * wpr + ((j - 1) >> 4)];
-            const unsigned d = (w >> (((j - 1) & 15) * 2)) & 3u;
-            if (d == 0) {
-                L.alP[k] = (short)(i - 1);
-                L.alN[k] = (short)(j - 1);
-                k++;
-                i--;
-                j--;
-            } else if (d == 1) {
-                i--;
-            } else {
-                j--;
+            const int top = i, c1 = (j - 1) >> 4;
+            const int r = top - 1 - lane;
+            unsigned w1 = 0, w0 = 0;
+            if (r >= 0) {
+                w1 = trace[(long long)r * wpr + c1];
+                if (c1 > 0) w0 = trace[(long long)r * wpr + c1 - 1];
+            }
+            while (i > 0 && j > 0) {
+                const int rl = top - i;
+                const int c = (j - 1) >> 4;
+                if (rl >= 64 || c < c1 - 1) break;
+                const unsigned a1 = (unsigned)__builtin_amdgcn_readlane((int)w1, rl);
+                const unsigned a0 = (unsigned)__builtin_amdgcn_readlane((int)w0, rl);
+                const unsigned w = c == c1 ? a1 : a0;
+                const unsigned d = (w >> (((j - 1) & 15) * 2)) & 3u;
+                if (d == 0) {
+                    if (lane == 0) {
+                        L.alP[k] = (short)(i - 1);
+                        L.alN[k] = (short)(j - 1);
+                    }
+                    k++;
+                    i--;
+                    j--;
+                } else if (d == 1) {
+                    i--;
+                } else {
+                    j--;
+                }
             }
         }
-        L.misc[0] = k;
+        if (lane == 0) L.misc[0] = k;
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -644,10 +688,11 @@ struct LdsPlan {
     }
 };
 
-// At most 256 VGPRs: two wavefronts per SIMD hide the LDS / DPP latency of the
-// fill; measured faster than one wavefront per SIMD with fewer spills.
+// At most 168 VGPRs: three wavefronts per SIMD hide the dependency latency of
+// the fill (measured 1.37x faster than one wavefront per SIMD on 40-300 nt
+// chains; the spills this costs sit outside the fill loop).
 template <bool FAST>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void chain_kernel(Arena A, const ChainGroup *groups,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void chain_kernel(Arena A, const ChainGroup *groups,
                                                    int n_groups, long long n_chains,
                                                    const Hit *xf, Hit *out, ListOut *lout,
                                                    int *lists_p, int *lists_n, int *list_na,
@@ -688,11 +733,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void ch
 
 }  // namespace
 
-int chain_blocks_per_cu(size_t lds) {
+int chain_blocks_per_cu(size_t lds, bool fast) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<false>, 64, lds) != hipSuccess)
-        return 1;
-    return nb;
+    const hipError_t e =
+        fast ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<true>, 64, lds)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<false>, 64, lds);
+    return e == hipSuccess ? nb : 1;
 }
 
 size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds) {
