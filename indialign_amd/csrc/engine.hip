@@ -549,7 +549,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     const ChainGroup *dg = upload(b_groups, groups);
     int *q = b_queue.get<int>(1);
     IA_CHECK(hipMemsetAsync(q, 0, sizeof(int), s_));
-    const int wpr = (mn + 15) / 16;
+    const int wpr = chain_trace_row_words(mn);
     const size_t trace_bytes = (size_t)mp * wpr * 4;
     // LDS layout: the trace and the staged columns each go to LDS only when
     // that costs no resident wavefront (the fill is latency-bound, so

@@ -4,16 +4,14 @@
 //   score matrix (fused, never stored) -> Needleman-Wunsch fill -> traceback
 //   -> Kabsch on the aligned rows -> detailed search -> keep the best.
 //
-// NW fill: rows are striped over the 64 lanes (lane l owns compact row
-// s0 + l of strip s0); the wavefront sweeps columns as a systolic pipeline,
-// lane l working on column t - l at step t, so every cell's up/diagonal
-// neighbours arrive from lane l-1 by a cross-lane shift and its left
-// neighbour is the lane's own previous result.  The last row of a strip is
-// handed to the next strip through an LDS row buffer.  Trace codes
-// (0 diag, 1 up, 2 left) are packed 16 cells per 32-bit word into LDS, or
+// NW fill: lane l computes rows l, l+64, l+128, ... one after the other; the
+// wavefront sweeps them as a systolic pipeline in which every cell's
+// up/diagonal neighbours arrive from lane l-1 by a cross-lane shift and its
+// left neighbour is the lane's own previous result.  Lane 0 takes the
+// previous round's last row from an LDS hand-over buffer.  Trace codes
+// (0 diag, 1 up, 2 left) are packed 16 steps per 32-bit word into LDS, or
 // into a per-wavefront global slot when the LDS copy would cost resident
-// wavefronts (long chains, or 40-300 nt batches at three wavefronts per
-// SIMD), and walked back from there.
+// wavefronts, and walked back from there.
 //
 // Work distribution: a persistent grid pulls chain indices from an atomic
 // queue, so chains of very different cost balance across the CUs.
@@ -40,198 +38,248 @@ IA_D double d0_mult(int g) {
 }
 
 struct ChainLds {
-    unsigned *trace;  // plen x wpr words (when the trace fits in LDS)
-    double *colx, *coly, *colz;  // the chain's compact native coordinates
+    unsigned *trace;  // trace words (when the trace fits in LDS)
+    double *colv;     // the chain's compact native coordinates, x y z per column
     char *cols;                  // compact native secondary structure
-    double *rowH;     // nlen
-    signed char *rowT;
-    short *alP, *alN; // aligned compact indices, reverse order
+    double *rowHG;    // row hand-over buffer: (H, G) of one row's nlen cells
+    short *alP, *alN; // aligned compact indices, reverse order (alias rowHG)
     int *misc;
 };
 
-// Cross-lane shift by one lane (DPP wave_shr:1, a VALU move; no LDS
-// round trip).  Lane 0 keeps its own value.
-IA_D int shr1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
-IA_D double shr1(double v) {
-    const int lo = shr1(__double2loint(v)), hi = shr1(__double2hiint(v));
+// Shift up by one lane (DPP wave_shr:1, a VALU move): lane l receives lane
+// l-1's v, lane 0 receives its own `old`.
+IA_D int shr1_old(int v, int old) { return __builtin_amdgcn_update_dpp(old, v, 0x138, 0xf, 0xf, false); }
+IA_D double shr1_old(double v, double old) {
+    const int lo = shr1_old(__double2loint(v), __double2loint(old));
+    const int hi = shr1_old(__double2hiint(v), __double2hiint(old));
     return __hiloint2double(hi, lo);
+}
+IA_D float shr1_old(float v, float old) {
+    return __int_as_float(shr1_old(__float_as_int(v), __float_as_int(old)));
 }
 
-// Rotation by one lane (DPP wave_ror:1): lane l receives lane l-1, lane 0
-// receives lane 63.
-IA_D int ror1(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x13C, 0xf, 0xf, false); }
-IA_D double ror1(double v) {
-    const int lo = ror1(__double2loint(v)), hi = ror1(__double2hiint(v));
-    return __hiloint2double(hi, lo);
+// max of two finite values as one v_max instruction.  fmax() would first
+// quiet possible signalling NaNs of operands that arrive through lane moves
+// (an extra v_max per operand); the fill never holds a NaN.
+IA_D double vmax(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
 }
-IA_D float ror1(float v) { return __int_as_float(ror1(__float_as_int(v))); }
+IA_D float vmax(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// 1 / y correctly rounded for 1 <= y < 2^1022: reciprocal seed, two Newton
+// steps and one remainder correction, the same operations as IEEE division
+// of 1 by y without the range scaling (v_div_scale leaves such y and 1.0
+// unscaled) and the special-case fix-up (none applies to a finite result).
+// Bit-identical to 1.0 / y there.
+IA_D double recip_small(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    r = fma(r, fma(-y, r, 1.0), r);
+    r = fma(r, fma(-y, r, 1.0), r);
+    return fma(fma(-y, r, 1.0), r, r);
+}
 
 // Needleman-Wunsch fill (nw.h:10-38) over the compact plen x nlen matrix,
 // score computed on the fly.
 //
-// The wavefront sweeps super-strips of 64*K rows.  Lane l owns rows
-// s0 + l + 64 j (slot j = 0..K-1) and, at step st, works on column
-// st - l - 64 j of each of them: the K cells of a lane are independent, so
-// their dependency chains overlap.  Up/diagonal operands come from lane l-1
-// by a lane rotation; lane 0 of slot j receives lane 63 of slot j-1 the same
-// way, and slot 0's lane 0 reads the previous super-strip's last row from
-// LDS.  Each cell's value H is max(diag, up, left) (ties only decide the
-// trace code), and G = H + gap_open when the cell was entered diagonally,
-// else H, is what its down and right neighbours add.
-template <int SMODE, class TR, bool CL, int K, class V>
+// Schedule: lane l computes rows l, l+64, l+128, ... one after another, a
+// row taking P = max(nlen, 65) steps, so row a = 64m + l holds cell (a, b)
+// at step s(a, b) = mP + l + b (fill_step).  Every cell's up and diagonal
+// operands were computed one step earlier by lane l-1 and arrive by a lane
+// shift; lane 0 reads the previous round's last row (lane 63's) from an LDS
+// buffer, which round 0 finds zeroed (the H = 0 boundary with no gap).  The
+// left operand is the lane's own previous value.  A round's first 64 steps
+// (phase A) are those where lane q = step - mP moves on to its next row;
+// the rest (phase B) need no per-lane row switch.
+//
+// Trace codes (0 diag, 1 up, 2 left) are packed by step: lane l shifts each
+// code into its word, and every 16 steps the wavefront stores one word per
+// lane to row a's slot (s >> 4) - (s(a, 0) >> 4) (a word that spans a row
+// switch goes to both rows).  H = max(diag, up, left) is exact; ties only
+// decide the trace code; G = H + gap_open for diagonally entered cells, else
+// H, is what down and right neighbours add.
+IA_HD int fill_period(int nlen) { return nlen > 65 ? nlen : 65; }
+IA_HD int trace_row_words(int nlen) { return ((nlen + 14) >> 4) + 1; }
+IA_HD long long fill_row_start(int a, int P) { return (long long)(a >> 6) * P + (a & 63); }
+
+template <int SMODE, class TR, bool CL, class V>
 IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const double *t,
                   double gap_d, double cut8_d, double d0sq_s, ChainLds &L, TR *trace) {
     const int lane = threadIdx.x & 63;
     const int plen = pi.plen, nlen = pi.nlen, off = pi.off;
-    const int wpr = (nlen + 15) >> 4;
     constexpr bool F32 = sizeof(V) == 4;
-    // column operands: FP32 copies (fast method), the wavefront's LDS copy,
-    // or global memory for chains too long to stage
-    auto colx = [&](int b) -> V { if constexpr (F32) return A.qfx[off + b]; else return CL ? L.colx[b] : A.qcx[off + b]; };
-    auto coly = [&](int b) -> V { if constexpr (F32) return A.qfy[off + b]; else return CL ? L.coly[b] : A.qcy[off + b]; };
-    auto colz = [&](int b) -> V { if constexpr (F32) return A.qfz[off + b]; else return CL ? L.colz[b] : A.qcz[off + b]; };
+    const int P = fill_period(nlen);
+    const int wpr = trace_row_words(nlen);
+    const int M = (plen + 63) >> 6;
+    const int lastl = plen - 1 - 64 * (M - 1);
+    auto colx = [&](int b) -> V { if constexpr (F32) return A.qfx[off + b]; else return CL ? L.colv[3 * b] : A.qcx[off + b]; };
+    auto coly = [&](int b) -> V { if constexpr (F32) return A.qfy[off + b]; else return CL ? L.colv[3 * b + 1] : A.qcy[off + b]; };
+    auto colz = [&](int b) -> V { if constexpr (F32) return A.qfz[off + b]; else return CL ? L.colv[3 * b + 2] : A.qcz[off + b]; };
     auto cols = [&](int b) { return (CL && !F32) ? L.cols[b] : A.ssn[off + A.nmap[off + b]]; };
     const V gap = (V)gap_d, cut8 = (V)cut8_d;
-    const V inv_d0sq = (V)(1.0 / d0sq_s);
-    V *rowHv = (V *)L.rowH;
-    for (int s0 = 0; s0 < plen; s0 += 64 * K) {
-        const int rows = min(64 * K, plen - s0);
-        const bool first = s0 == 0;
-        const bool has_next = s0 + 64 * K < plen;
-        V mx[K], my[K], mz[K], H[K], G[K], D[K], cx[K], cy[K], cz[K];
-        char sp[K], cs[K];
-        bool rok[K];
-        unsigned tw[K];
-#pragma unroll
-        for (int j = 0; j < K; j++) {
-            const int a = s0 + 64 * j + lane;
-            rok[j] = a < plen;
-            mx[j] = my[j] = mz[j] = 0.0;
-            sp[j] = 0;
-            if (rok[j]) {
-                if (SMODE != SM_SS) {
-                    double ex, ey, ez;
-                    xform(R, t, A.pcx[off + a], A.pcy[off + a], A.pcz[off + a], ex, ey, ez);
-                    mx[j] = (V)ex; my[j] = (V)ey; mz[j] = (V)ez;
-                }
-                if (SMODE != SM_TM) sp[j] = A.ssp[off + A.pmap[off + a]];
-            }
-            H[j] = G[j] = D[j] = 0.0;
-            tw[j] = 0;
-            const int b = min(max(-64 * j - lane, 0), nlen - 1);
-            cx[j] = cy[j] = cz[j] = 0.0;
-            cs[j] = 0;
-            if (SMODE != SM_SS) { cx[j] = colx(b); cy[j] = coly(b); cz[j] = colz(b); }
-            if (SMODE != SM_TM) cs[j] = cols(b);
+    const double inv_d0sq_d = 1.0 / d0sq_s;
+    const V inv_d0sq = (V)inv_d0sq_d;
+    typedef typename std::conditional<F32, float2, double2>::type V2;
+    V2 *rowHG = (V2 *)L.rowHG;
+    for (int b = lane; b < nlen; b += 64) rowHG[b] = V2{(V)0, (V)0};
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+
+    // transformed coordinates / SS code of the lane's next row
+    V nmx = 0, nmy = 0, nmz = 0;
+    char nsp = 0;
+    auto next_row = [&](int r) {
+        const int rc = min(r, plen - 1);
+        if (SMODE != SM_SS) {
+            double ex, ey, ez;
+            xform(R, t, A.pcx[off + rc], A.pcy[off + rc], A.pcz[off + rc], ex, ey, ez);
+            nmx = (V)ex; nmy = (V)ey; nmz = (V)ez;
         }
-        const int steps = nlen + rows - 1;
-        V rH = first ? (V)0 : rowHv[0];
-        int rT = first ? -1 : (int)L.rowT[0];
-        for (int st = 0; st < steps; st++) {
-            const int rn = min(st + 1, nlen - 1);
-            const V rHn = first ? (V)0 : rowHv[rn];
-            const int rTn = first ? -1 : (int)L.rowT[rn];
-            V XH[K], XG[K];
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                XH[j] = ror1(H[j]);
-                XG[j] = ror1(G[j]);
+        if (SMODE != SM_TM) nsp = A.ssp[off + A.pmap[off + rc]];
+    };
+    next_row(lane);
+    V mx = 0, my = 0, mz = 0;
+    char sp = 0;
+    int a = lane - 64, b = P - lane;        // current row and column (lane l
+                                            // reaches b = P, its first row, at step l)
+    int start = 0, a_old = -1, start_old = 0;  // first step of the current / previous row
+    bool sww = false;                       // switched rows within the current trace word
+    V H = 0, G = 0, D = 0;
+    unsigned tw = 0;
+    // operands of step 0: lane 0 is on column 0, the others are idle
+    V cx = 0, cy = 0, cz = 0;
+    char cs = 0;
+    if (SMODE != SM_SS) { cx = colx(0); cy = coly(0); cz = colz(0); }
+    if (SMODE != SM_TM) cs = cols(0);
+    V rH = 0, rG = 0;
+    int st = 0;
+
+    auto flush = [&](int nsteps) {
+        const unsigned w = nsteps == 16 ? tw : tw >> (32 - 2 * nsteps);
+        const int w0 = st - nsteps + 1;  // first step of the word
+        if ((unsigned)a < (unsigned)plen && w0 < start + nlen)
+            trace[(long long)a * wpr + ((st >> 4) - (start >> 4))] = w;
+        if (sww && (unsigned)a_old < (unsigned)plen && w0 < start_old + nlen)
+            trace[(long long)a_old * wpr + ((st >> 4) - (start_old >> 4))] = w;
+        sww = false;
+    };
+
+    // PA: phase A (lane q starts its next row).  GC: the general next-column
+    // rule; without it (phase B, not a round's last step) lane l's next
+    // column is q + 1 - l.
+    auto step = [&](auto phase_a, auto general_col, int q, int q_next) {
+        constexpr bool PA = decltype(phase_a)::value;
+        constexpr bool GC = decltype(general_col)::value;
+        const V hu = shr1_old(H, rH);
+        const V gu = shr1_old(G, rG);
+        V Dc = D, Gl = G;
+        if constexpr (PA) {
+            const bool e = lane == q;
+            if (e) {
+                a_old = a; start_old = start; sww = true;
+                a += 64; b = 0; start = st;
+                mx = nmx; my = nmy; mz = nmz; sp = nsp;
             }
-            // FP32: the next step's columns flow down the lanes with the
-            // cells (lane l's next column is lane l-1's current one); only
-            // slot 0's lane 0 reads a new column, at a wave-uniform address.
-            V Xx[K], Xy[K], Xz[K], c0x = 0, c0y = 0, c0z = 0;
+            Dc = e ? (V)0 : D;
+            Gl = e ? (V)0 : G;
+        }
+        const bool act = (unsigned)a < (unsigned)plen && b < nlen;
+        // next step's column operands (prefetch)
+        const int rq = min(q_next, nlen - 1);  // lane 0's next column, clamped
+        int bn;
+        if constexpr (GC) bn = b + 1 >= P ? 0 : min(b + 1, nlen - 1);
+        else bn = min(q + 1 - lane, nlen - 1);
+        V nx = 0, ny = 0, nz = 0;
+        char ns = 0;
+        if (SMODE != SM_SS) {
             if constexpr (F32) {
-                if (SMODE != SM_SS) {
-#pragma unroll
-                    for (int j = 0; j < K; j++) {
-                        Xx[j] = ror1(cx[j]);
-                        Xy[j] = ror1(cy[j]);
-                        Xz[j] = ror1(cz[j]);
-                    }
-                    c0x = colx(rn);
-                    c0y = coly(rn);
-                    c0z = colz(rn);
-                }
+                // lane l's next column is lane l-1's current one; lane 0's is
+                // q_next, read at a wave-uniform address
+                nx = shr1_old(cx, colx(rq));
+                ny = shr1_old(cy, coly(rq));
+                nz = shr1_old(cz, colz(rq));
+            } else {
+                nx = colx(bn); ny = coly(bn); nz = colz(bn);
             }
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                const int b = st - 64 * j - lane;
-                const bool act = rok[j] && b >= 0 && b < nlen;
-                // next step's column operands
-                const int bn = min(max(b + 1, 0), nlen - 1);
-                V nx = 0, ny = 0, nz = 0;
-                char ns = 0;
-                if (SMODE != SM_SS) {
-                    if constexpr (F32) {
-                        nx = lane == 0 ? (j == 0 ? c0x : Xx[j > 0 ? j - 1 : 0]) : Xx[j];
-                        ny = lane == 0 ? (j == 0 ? c0y : Xy[j > 0 ? j - 1 : 0]) : Xy[j];
-                        nz = lane == 0 ? (j == 0 ? c0z : Xz[j > 0 ? j - 1 : 0]) : Xz[j];
-                    } else {
-                        nx = colx(bn); ny = coly(bn); nz = colz(bn);
-                    }
-                }
-                if (SMODE != SM_TM) ns = cols(bn);
-                V s;
-                if (SMODE == SM_SS) {
-                    s = (sp[j] == cs[j]) ? (V)1 : (V)0;
-                } else {
-                    const V dx = mx[j] - cx[j], dy = my[j] - cy[j], dz = mz[j] - cz[j];
-                    V d2, tm;
-                    if constexpr (F32) {
-                        d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                        tm = __builtin_amdgcn_rcpf(__builtin_fmaf(d2, inv_d0sq, 1.0f));
-                    } else {
-                        d2 = dx * dx + dy * dy + dz * dz;
-                        tm = 1.0 / (1.0 + div_by(d2, d0sq_s, inv_d0sq));
-                    }
-                    if (SMODE == SM_TM) {
-                        s = (d2 <= cut8) ? tm : (V)0;
-                    } else {
-                        s = tm;
-                        if (sp[j] == cs[j]) s += (V)0.5;
-                    }
-                }
-                // up operands (raw H for the next diagonal, G for "up")
-                V hu, gu;
-                if (j == 0) {
-                    const bool bound = first || b >= nlen;
-                    const V h0 = bound ? (V)0 : rH;
-                    const V g0 = bound ? (V)0 : ((rT == 0) ? rH + gap : rH);
-                    hu = lane == 0 ? h0 : XH[0];
-                    gu = lane == 0 ? g0 : XG[0];
-                } else {
-                    hu = lane == 0 ? XH[j > 0 ? j - 1 : 0] : XH[j];
-                    gu = lane == 0 ? XG[j > 0 ? j - 1 : 0] : XG[j];
-                }
-                const bool edge = b == 0;
-                const V diag = (edge ? (V)0 : D[j]) + s;
-                const V left = edge ? (V)0 : G[j];
-                const V hm = fmax(diag, fmax(gu, left));
-                const int tn = (diag >= gu && diag >= left) ? 0 : (left >= gu ? 2 : 1);
-                const V gn = (tn == 0) ? hm + gap : hm;
-                if (act) {
-                    tw[j] |= (unsigned)tn << ((b & 15) * 2);
-                    if ((b & 15) == 15 || b == nlen - 1) {
-                        trace[(long long)(s0 + 64 * j + lane) * wpr + (b >> 4)] = tw[j];
-                        tw[j] = 0;
-                    }
-                    if (j == K - 1 && lane == 63 && has_next) {
-                        rowHv[b] = hm;
-                        L.rowT[b] = (signed char)tn;
-                    }
-                }
-                D[j] = hu;
-                H[j] = hm;
-                G[j] = gn;
-                cx[j] = nx; cy[j] = ny; cz[j] = nz; cs[j] = ns;
-            }
-            rH = rHn;
-            rT = rTn;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
+        if (SMODE != SM_TM) ns = cols(bn);
+        const V2 rn = rowHG[rq];
+        V s;
+        if (SMODE == SM_SS) {
+            s = (sp == cs) ? (V)1 : (V)0;
+        } else {
+            const V dx = mx - cx, dy = my - cy, dz = mz - cz;
+            V d2, tm;
+            if constexpr (F32) {
+                d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                tm = __builtin_amdgcn_rcpf(__builtin_fmaf(d2, inv_d0sq, 1.0f));
+            } else {
+                d2 = dx * dx + dy * dy + dz * dz;
+                const double y = 1.0 + div_by(d2, d0sq_s, inv_d0sq_d);
+                // TM mode keeps only cells with d2 <= cut8, whose y is far
+                // below 2^1022; IA4 scores every cell, so it divides
+                tm = SMODE == SM_TM ? recip_small(y) : 1.0 / y;
+            }
+            if (SMODE == SM_TM) {
+                s = (d2 <= cut8) ? tm : (V)0;
+            } else {
+                s = tm;
+                if (sp == cs) s += (V)0.5;
+            }
+        }
+        const V diag = Dc + s;
+        const V mgl = vmax(gu, Gl);
+        const V hm = vmax(diag, mgl);
+        const bool cD = diag >= mgl;
+        const unsigned tn = cD ? 0u : (Gl >= gu ? 2u : 1u);
+        const V gn = cD ? diag + gap : mgl;
+        if (lane == 63 && act) rowHG[b] = V2{hm, gn};  // to the next round's lane 0
+        tw = __builtin_amdgcn_alignbit(tn, tw, 2);
+        D = hu; H = hm; G = gn;
+        cx = nx; cy = ny; cz = nz; cs = ns;
+        rH = rn.x; rG = rn.y;
+        if ((st & 15) == 15) flush(16);
+        st++;
+        b++;
+    };
+
+    using TA = std::true_type;
+    using TB = std::false_type;
+    for (int m = 0; m < M; m++) {
+        const bool last = m == M - 1;
+        const int qend = last ? lastl + nlen : P;
+        const int qa = min(64, qend);
+        for (int q = 0; q < qa; q++) step(TA(), TA(), q, q + 1 < qend ? q + 1 : 0);
+        // phase B, two steps per iteration (lets loop-carried values
+        // alternate registers instead of being copied back every step)
+        const int qb = last ? qend : P - 1;
+        if (!last) next_row(64 * (m + 1) + lane);
+        int q = qa;
+        for (; q + 1 < qb; q += 2) {
+            step(TB(), TB(), q, q + 1);
+            step(TB(), TB(), q + 1, q + 2);
+        }
+        if (q < qb) step(TB(), TB(), q, q + 1);
+        if (!last) step(TB(), TA(), P - 1, 0);
     }
+    if (st & 15) {  // the last, partial word
+        st--;
+        flush((st & 15) + 1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Trace word of cell (a, b) and the code's bit position in it.
+IA_D long long trace_word_index(int a, int b, int P, int wpr, int &shift) {
+    const long long s0 = fill_row_start(a, P), s = s0 + b;
+    shift = (int)(s & 15) * 2;
+    return (long long)a * wpr + ((s >> 4) - (s0 >> 4));
 }
 
 // Needleman-Wunsch fill + traceback; returns the aligned count (pairs in
@@ -241,12 +289,10 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
                   double gap, double cut8, double d0sq_s, ChainLds &L, TR *trace) {
     const int lane = threadIdx.x & 63;
     const int plen = pi.plen, nlen = pi.nlen;
-    const int wpr = (nlen + 15) >> 4;
+    const int P = fill_period(nlen);
+    const int wpr = trace_row_words(nlen);
     typedef typename std::conditional<FAST, float, double>::type V;
-    if (plen <= 64)
-        nw_fill<SMODE, TR, CL, 1, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
-    else
-        nw_fill<SMODE, TR, CL, 2, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
+    nw_fill<SMODE, TR, CL, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
     __builtin_amdgcn_wave_barrier();
     // traceback (nw.h:42-68)
@@ -255,8 +301,9 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
         if (lane == 0) {
             int i = plen, j = nlen, k = 0;
             while (i > 0 && j > 0) {
-                const unsigned w = trace[(long long)(i - 1) * wpr + ((j - 1) >> 4)];
-                const unsigned d = (w >> (((j - 1) & 15) * 2)) & 3u;
+                int sh;
+                const unsigned w = trace[trace_word_index(i - 1, j - 1, P, wpr, sh)];
+                const unsigned d = (w >> sh) & 3u;
                 if (d == 0) {
                     L.alP[k] = (short)(i - 1);
                     L.alN[k] = (short)(j - 1);
@@ -273,27 +320,33 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
         }
     } else {
         // trace in global memory: one round of loads brings a window of 64
-        // rows (lane l holds row top-1-l) by two words (the current word
-        // column c1 and c1-1) into registers; the wavefront walks it with
-        // wave-uniform (i, j) and fetches each step's word by readlane,
-        // reloading when the path leaves the window.
+        // rows (lane l holds row top-1-l) by two words (the word holding the
+        // current column and the one before it) into registers; the
+        // wavefront walks it with wave-uniform (i, j) and fetches each step's
+        // word by readlane, reloading when the path leaves the window.
         int i = plen, j = nlen, k = 0;
         while (i > 0 && j > 0) {
-            const int top = i, c1 = (j - 1) >> 4;
+            const int top = i;
             const int r = top - 1 - lane;
             unsigned w1 = 0, w0 = 0;
+            int kk = 0;
             if (r >= 0) {
-                w1 = trace[(long long)r * wpr + c1];
-                if (c1 > 0) w0 = trace[(long long)r * wpr + c1 - 1];
+                const long long s0 = fill_row_start(r, P);
+                kk = (int)(((s0 + j - 1) >> 4) - (s0 >> 4));
+                w1 = trace[(long long)r * wpr + kk];
+                if (kk > 0) w0 = trace[(long long)r * wpr + kk - 1];
             }
             while (i > 0 && j > 0) {
                 const int rl = top - i;
-                const int c = (j - 1) >> 4;
-                if (rl >= 64 || c < c1 - 1) break;
+                if (rl >= 64) break;
+                const long long s0 = fill_row_start(i - 1, P), s = s0 + j - 1;
+                const int kw = (int)((s >> 4) - (s0 >> 4));
+                const int kr = __builtin_amdgcn_readlane(kk, rl);
+                if (kw != kr && kw != kr - 1) break;
                 const unsigned a1 = (unsigned)__builtin_amdgcn_readlane((int)w1, rl);
                 const unsigned a0 = (unsigned)__builtin_amdgcn_readlane((int)w0, rl);
-                const unsigned w = c == c1 ? a1 : a0;
-                const unsigned d = (w >> (((j - 1) & 15) * 2)) & 3u;
+                const unsigned w = kw == kr ? a1 : a0;
+                const unsigned d = (w >> ((int)(s & 15) * 2)) & 3u;
                 if (d == 0) {
                     if (lane == 0) {
                         L.alP[k] = (short)(i - 1);
@@ -467,9 +520,9 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     const int off = pi.off;
     for (int b = lane; CL && b < pi.nlen; b += 64) {
         if (SMODE != SM_SS) {
-            L.colx[b] = A.qcx[off + b];
-            L.coly[b] = A.qcy[off + b];
-            L.colz[b] = A.qcz[off + b];
+            L.colv[3 * b] = A.qcx[off + b];
+            L.colv[3 * b + 1] = A.qcy[off + b];
+            L.colv[3 * b + 2] = A.qcz[off + b];
         }
         if (SMODE != SM_TM) L.cols[b] = A.ssn[off + A.nmap[off + b]];
     }
@@ -649,7 +702,9 @@ IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, lon
     while (true) {
         long long task = 0;
         if (lane == 0) task = atomicAdd(queue, 1);
-        task = __shfl(task, 0, 64);
+        // lane 0's value in scalar registers: everything derived from the
+        // chain index (group, pair, sizes, loop bounds) is then wave-uniform
+        task = (long long)__builtin_amdgcn_readfirstlane((int)task);
         if (task >= n_chains) break;
         const int gi = find_group(groups, n_groups, task);
         const ChainGroup &G = groups[gi];
@@ -666,24 +721,23 @@ IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, lon
     }
 }
 
-// LDS carve-out of one wavefront (shared with chain_lds_bytes).
+// LDS carve-out of one wavefront (shared with chain_lds_bytes).  The
+// traceback's aligned-pair lists reuse the row hand-over buffer, which is
+// dead once the fill has finished (and is cleared by the next fill).
 struct LdsPlan {
-    size_t rowH, misc, rowT, alP, alN, colx, coly, colz, cols, trace, total;
+    size_t rowHG, misc, alP, alN, colv, cols, trace, total;
     IA_HD LdsPlan(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds) {
-        const size_t mn = (size_t)(max_plen < max_nlen ? max_plen : max_nlen);
+        (void)max_plen;
         const size_t n = (size_t)max_nlen, nc = cols_in_lds ? n : 0;
         size_t o = 0;
-        rowH = o; o += 8 * n;
-        colx = o; o += 8 * nc;
-        coly = o; o += 8 * nc;
-        colz = o; o += 8 * nc;
+        rowHG = o; o += 16 * n;
+        alP = rowHG;  // 2 min(plen, nlen) <= 8 nlen bytes
+        alN = rowHG + 8 * n;
+        colv = o; o += 24 * nc;
         misc = o; o += 16;
-        rowT = o; o += (n + 15) & ~(size_t)15;
         cols = o; o += (nc + 15) & ~(size_t)15;
-        alP = o; o += (2 * mn + 15) & ~(size_t)15;
-        alN = o; o += (2 * mn + 15) & ~(size_t)15;
         trace = o;
-        if (trace_in_lds) o += (size_t)max_plen * (size_t)((max_nlen + 15) >> 4) * 4;
+        if (trace_in_lds) o += (size_t)max_plen * (size_t)trace_row_words(max_nlen) * 4;
         total = o;
     }
 };
@@ -702,12 +756,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ch
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsPlan P(max_plen, max_nlen, trace_g == nullptr, cols_in_lds != 0);
     ChainLds L;
-    L.rowH = (double *)(lds + P.rowH);
-    L.colx = (double *)(lds + P.colx);
-    L.coly = (double *)(lds + P.coly);
-    L.colz = (double *)(lds + P.colz);
+    L.rowHG = (double *)(lds + P.rowHG);
+    L.colv = (double *)(lds + P.colv);
     L.misc = (int *)(lds + P.misc);
-    L.rowT = (signed char *)(lds + P.rowT);
     L.cols = (char *)(lds + P.cols);
     L.alP = (short *)(lds + P.alP);
     L.alN = (short *)(lds + P.alN);
@@ -740,6 +791,8 @@ int chain_blocks_per_cu(size_t lds, bool fast) {
              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<false>, 64, lds);
     return e == hipSuccess ? nb : 1;
 }
+
+int chain_trace_row_words(int nlen) { return trace_row_words(nlen); }
 
 size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds) {
     return LdsPlan(max_plen, max_nlen, trace_in_lds, cols_in_lds).total;
