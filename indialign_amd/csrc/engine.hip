@@ -564,7 +564,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     const int occ_free = occupancy(false, false);
     if (occ_free <= 0) throw DevError("chain too long for one workgroup's LDS");
     const bool in_lds = trace_bytes <= 96 * 1024 && occupancy(true, false) >= occ_free;
-    const bool cols_lds = !fast_ && occupancy(in_lds, true) >= occupancy(in_lds, false);
+    const bool cols_lds = occupancy(in_lds, true) >= occupancy(in_lds, false);
     const size_t lds = chain_lds_bytes(mp, mn, in_lds, cols_lds);
     int per_cu = std::max(1, chain_blocks_per_cu(lds, fast_));
     per_cu = std::min(per_cu, 16);

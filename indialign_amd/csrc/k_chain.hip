@@ -117,10 +117,13 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     const int wpr = trace_row_words(nlen);
     const int M = (plen + 63) >> 6;
     const int lastl = plen - 1 - 64 * (M - 1);
-    auto colx = [&](int b) -> V { if constexpr (F32) return A.qfx[off + b]; else return CL ? L.colv[3 * b] : A.qcx[off + b]; };
-    auto coly = [&](int b) -> V { if constexpr (F32) return A.qfy[off + b]; else return CL ? L.colv[3 * b + 1] : A.qcy[off + b]; };
-    auto colz = [&](int b) -> V { if constexpr (F32) return A.qfz[off + b]; else return CL ? L.colv[3 * b + 2] : A.qcz[off + b]; };
-    auto cols = [&](int b) { return (CL && !F32) ? L.cols[b] : A.ssn[off + A.nmap[off + b]]; };
+    // column operands: the wavefront's LDS copy (FP64, or FP32 for the fast
+    // method) or, for chains too long to stage, the arena
+    const V *colv = (const V *)L.colv;
+    auto colx = [&](int b) -> V { if constexpr (F32) return CL ? colv[3 * b] : A.qfx[off + b]; else return CL ? colv[3 * b] : A.qcx[off + b]; };
+    auto coly = [&](int b) -> V { if constexpr (F32) return CL ? colv[3 * b + 1] : A.qfy[off + b]; else return CL ? colv[3 * b + 1] : A.qcy[off + b]; };
+    auto colz = [&](int b) -> V { if constexpr (F32) return CL ? colv[3 * b + 2] : A.qfz[off + b]; else return CL ? colv[3 * b + 2] : A.qcz[off + b]; };
+    auto cols = [&](int b) { return CL ? L.cols[b] : A.ssn[off + A.nmap[off + b]]; };
     const V gap = (V)gap_d, cut8 = (V)cut8_d;
     const double inv_d0sq_d = 1.0 / d0sq_s;
     const V inv_d0sq = (V)inv_d0sq_d;
@@ -151,11 +154,19 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     bool sww = false;                       // switched rows within the current trace word
     V H = 0, G = 0, D = 0;
     unsigned tw = 0;
-    // operands of step 0: lane 0 is on column 0, the others are idle
-    V cx = 0, cy = 0, cz = 0;
-    char cs = 0;
-    if (SMODE != SM_SS) { cx = colx(0); cy = coly(0); cz = colz(0); }
-    if (SMODE != SM_TM) cs = cols(0);
+    // column operands of the current step.  FP64: loaded per lane one step
+    // ahead.  FP32: lane l's column is the one lane l-1 had a step earlier,
+    // so they move down the lanes at the start of each step and only lane
+    // 0's (c0*) is loaded, one step ahead, at a wave-uniform address.
+    V cx = 0, cy = 0, cz = 0, c0x = 0, c0y = 0, c0z = 0;
+    char cs = 0, c0s = 0;
+    if constexpr (F32) {
+        if (SMODE != SM_SS) { c0x = colx(0); c0y = coly(0); c0z = colz(0); }
+        if (SMODE != SM_TM) c0s = cols(0);
+    } else {  // step 0: lane 0 is on column 0, the others are idle
+        if (SMODE != SM_SS) { cx = colx(0); cy = coly(0); cz = colz(0); }
+        if (SMODE != SM_TM) cs = cols(0);
+    }
     V rH = 0, rG = 0;
     int st = 0;
 
@@ -177,6 +188,14 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         constexpr bool GC = decltype(general_col)::value;
         const V hu = shr1_old(H, rH);
         const V gu = shr1_old(G, rG);
+        if constexpr (F32) {
+            if (SMODE != SM_SS) {
+                cx = shr1_old(cx, c0x);
+                cy = shr1_old(cy, c0y);
+                cz = shr1_old(cz, c0z);
+            }
+            if (SMODE != SM_TM) cs = (char)shr1_old((int)cs, (int)c0s);
+        }
         V Dc = D, Gl = G;
         if constexpr (PA) {
             const bool e = lane == q;
@@ -191,23 +210,18 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         const bool act = (unsigned)a < (unsigned)plen && b < nlen;
         // next step's column operands (prefetch)
         const int rq = min(q_next, nlen - 1);  // lane 0's next column, clamped
-        int bn;
-        if constexpr (GC) bn = b + 1 >= P ? 0 : min(b + 1, nlen - 1);
-        else bn = min(q + 1 - lane, nlen - 1);
         V nx = 0, ny = 0, nz = 0;
         char ns = 0;
-        if (SMODE != SM_SS) {
-            if constexpr (F32) {
-                // lane l's next column is lane l-1's current one; lane 0's is
-                // q_next, read at a wave-uniform address
-                nx = shr1_old(cx, colx(rq));
-                ny = shr1_old(cy, coly(rq));
-                nz = shr1_old(cz, colz(rq));
-            } else {
-                nx = colx(bn); ny = coly(bn); nz = colz(bn);
-            }
+        if constexpr (F32) {
+            if (SMODE != SM_SS) { c0x = colx(rq); c0y = coly(rq); c0z = colz(rq); }
+            if (SMODE != SM_TM) c0s = cols(rq);
+        } else {
+            int bn;
+            if constexpr (GC) bn = b + 1 >= P ? 0 : min(b + 1, nlen - 1);
+            else bn = min(q + 1 - lane, nlen - 1);
+            if (SMODE != SM_SS) { nx = colx(bn); ny = coly(bn); nz = colz(bn); }
+            if (SMODE != SM_TM) ns = cols(bn);
         }
-        if (SMODE != SM_TM) ns = cols(bn);
         const V2 rn = rowHG[rq];
         V s;
         if (SMODE == SM_SS) {
@@ -241,7 +255,7 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         if (lane == 63 && act) rowHG[b] = V2{hm, gn};  // to the next round's lane 0
         tw = __builtin_amdgcn_alignbit(tn, tw, 2);
         D = hu; H = hm; G = gn;
-        cx = nx; cy = ny; cz = nz; cs = ns;
+        if constexpr (!F32) { cx = nx; cy = ny; cz = nz; cs = ns; }
         rH = rn.x; rG = rn.y;
         if ((st & 15) == 15) flush(16);
         st++;
@@ -520,9 +534,16 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     const int off = pi.off;
     for (int b = lane; CL && b < pi.nlen; b += 64) {
         if (SMODE != SM_SS) {
-            L.colv[3 * b] = A.qcx[off + b];
-            L.colv[3 * b + 1] = A.qcy[off + b];
-            L.colv[3 * b + 2] = A.qcz[off + b];
+            if (FAST) {
+                float *cf = (float *)L.colv;
+                cf[3 * b] = A.qfx[off + b];
+                cf[3 * b + 1] = A.qfy[off + b];
+                cf[3 * b + 2] = A.qfz[off + b];
+            } else {
+                L.colv[3 * b] = A.qcx[off + b];
+                L.colv[3 * b + 1] = A.qcy[off + b];
+                L.colv[3 * b + 2] = A.qcz[off + b];
+            }
         }
         if (SMODE != SM_TM) L.cols[b] = A.ssn[off + A.nmap[off + b]];
     }
@@ -809,7 +830,7 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
     if (fast)
         hipLaunchKernelGGL(chain_kernel<true>, dim3(grid), dim3(64), lds, s, A, groups, n_groups,
                            n_chains, xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen,
-                           max_nlen, queue, trace_scratch, trace_slot_words, 0);
+                           max_nlen, queue, trace_scratch, trace_slot_words, (int)cols_in_lds);
     else
         hipLaunchKernelGGL(chain_kernel<false>, dim3(grid), dim3(64), lds, s, A, groups, n_groups,
                            n_chains, xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen,
