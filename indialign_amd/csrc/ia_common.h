@@ -98,13 +98,51 @@ IA_HD double dist2_moved_fma(const double *R, const double *t, double x, double 
 // Cyclic Jacobi, 8 sweeps, pair order (0,1)(0,2)(0,3)(1,2)(1,3)(2,3),
 // rotation skipped when |a_pq| < 1e-15 (core.h:29-60).  Fully unrolled so
 // every index is a compile-time constant (the matrices stay in registers).
+//
+// On the device the square roots and the 1 / x divides run without the
+// range scaling and special-case fix-up of the IEEE sequences wherever the
+// argument is known to be in range (the remaining operations are the same,
+// so the results are bit-identical): 1 + tn^2 is in [1, 2], and
+// 1 + theta^2 is finite below 2^1022 for |theta| < 2^511.
+
+// sqrt(x), correctly rounded, for 2^-767 <= x < 2^1022: reciprocal square
+// root seed, one coupled step for sqrt and 1/(2 sqrt), two Newton corrections.
+IA_D double sqrt_in_range(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    g = fma(fma(-g, g, x), h, g);
+    return fma(fma(-g, g, x), h, g);
+}
+// 1 / y, correctly rounded, for 1 <= y < 2^1022: reciprocal seed, two Newton
+// steps, one remainder correction.
+IA_D double recip_in_range(double y) {
+    double r = __builtin_amdgcn_rcp(y);
+    r = fma(r, fma(-y, r, 1.0), r);
+    r = fma(r, fma(-y, r, 1.0), r);
+    return fma(fma(-y, r, 1.0), r, r);
+}
+
 template <int P, int Q>
 IA_HD void jacobi_rot(double a[4][4], double v[4][4]) {
     const double apq = a[P][Q];
     if (fabs(apq) < 1e-15) return;
     const double theta = (a[Q][Q] - a[P][P]) / (2.0 * apq);
+#ifdef __HIP_DEVICE_COMPILE__
+    double tn;
+    if (fabs(theta) < 0x1p511) {
+        const double r = recip_in_range(fabs(theta) + sqrt_in_range(1.0 + theta * theta));
+        tn = theta >= 0 ? r : -r;
+    } else {
+        tn = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+    }
+    const double c = recip_in_range(sqrt_in_range(1.0 + tn * tn));
+#else
     const double tn = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
     const double c = 1.0 / sqrt(1.0 + tn * tn);
+#endif
     const double s = tn * c;
     a[P][P] -= tn * apq;
     a[Q][Q] += tn * apq;

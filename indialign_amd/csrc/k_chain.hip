@@ -72,18 +72,6 @@ IA_D float vmax(float a, float b) {
     return r;
 }
 
-// 1 / y correctly rounded for 1 <= y < 2^1022: reciprocal seed, two Newton
-// steps and one remainder correction, the same operations as IEEE division
-// of 1 by y without the range scaling (v_div_scale leaves such y and 1.0
-// unscaled) and the special-case fix-up (none applies to a finite result).
-// Bit-identical to 1.0 / y there.
-IA_D double recip_small(double y) {
-    double r = __builtin_amdgcn_rcp(y);
-    r = fma(r, fma(-y, r, 1.0), r);
-    r = fma(r, fma(-y, r, 1.0), r);
-    return fma(fma(-y, r, 1.0), r, r);
-}
-
 // Needleman-Wunsch fill (nw.h:10-38) over the compact plen x nlen matrix,
 // score computed on the fly.
 //
@@ -237,7 +225,7 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
                 const double y = 1.0 + div_by(d2, d0sq_s, inv_d0sq_d);
                 // TM mode keeps only cells with d2 <= cut8, whose y is far
                 // below 2^1022; IA4 scores every cell, so it divides
-                tm = SMODE == SM_TM ? recip_small(y) : 1.0 / y;
+                tm = SMODE == SM_TM ? recip_in_range(y) : 1.0 / y;
             }
             if (SMODE == SM_TM) {
                 s = (d2 <= cut8) ? tm : (V)0;
@@ -385,7 +373,16 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
 
 // Lite detailed search (align.h:15-50 with use_frag = false): the whole
 // aligned set as the only seed, cutoffs d0s x {2,1.5,1,.75,.5,.25}, 8
-// refinement rounds, TM-score with the d8 cut.  Eight lanes per cutoff.
+// refinement rounds, TM-score with the d8 cut.  Eight lanes per cutoff; lane
+// s of a group takes aligned pairs s, s+8, s+16, ... in that order.
+//
+// The passes over the aligned pairs are latency-bound gathers, so each lane
+// loads four pairs (32-byte AoS records) before using any of them, and the
+// sums take every pair: a pair outside the cutoff adds +0.0.  The sums start
+// at +0.0 and so never hold -0.0, which makes adding +0.0 the same as
+// skipping the pair, bit for bit.
+constexpr int kLiteBatch = 4;
+
 template <bool FAST>
 IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int na,
                       const double *nxR, const double *nxt, double sd8, Hit &det) {
@@ -399,6 +396,23 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
 #pragma unroll
     for (int q = 0; q < 9; q++) R[q] = nxR[q];
     t[0] = nxt[0]; t[1] = nxt[1]; t[2] = nxt[2];
+    // the aligned pairs of one batch: lane s takes pairs base + s + 8k
+    struct Batch {
+        double x[kLiteBatch], y[kLiteBatch], z[kLiteBatch];
+        double a[kLiteBatch], b[kLiteBatch], c[kLiteBatch];
+        bool in[kLiteBatch];
+    };
+    auto load = [&](int base, bool on, Batch &B) {
+#pragma unroll
+        for (int k = 0; k < kLiteBatch; k++) {
+            const int q = base + s + 8 * k;
+            B.in[k] = on && q < na;
+            const int qq = q < na ? q : 0;  // na >= 3
+            const P4 P = A.pc4[off + L.alP[qq]], Q = A.qc4[off + L.alN[qq]];
+            B.x[k] = P.x; B.y[k] = P.y; B.z[k] = P.z;
+            B.a[k] = Q.x; B.b[k] = Q.y; B.c[k] = Q.z;
+        }
+    };
     bool done = !gact;
     int last = -1;
     for (int it = 0; it < 8; it++) {
@@ -407,18 +421,20 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
         int cnt = 0;
         if (FAST) {  // one pass: selection + raw second moments
             double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            if (!done) {
-                for (int q = s; q < na; q += 8) {
-                    const int ap = L.alP[q], an = L.alN[q];
-                    const double x = A.pcx[off + ap], y = A.pcy[off + ap], z = A.pcz[off + ap];
-                    const double qa = A.qcx[off + an], qb = A.qcy[off + an], qc = A.qcz[off + an];
-                    if (dist2_moved_fma(R, t, x, y, z, qa, qb, qc) <= lim) {
-                        cnt++;
-                        c[0] += x; c[1] += y; c[2] += z; c[3] += qa; c[4] += qb; c[5] += qc;
-                        h[0] = fma(x, qa, h[0]); h[1] = fma(x, qb, h[1]); h[2] = fma(x, qc, h[2]);
-                        h[3] = fma(y, qa, h[3]); h[4] = fma(y, qb, h[4]); h[5] = fma(y, qc, h[5]);
-                        h[6] = fma(z, qa, h[6]); h[7] = fma(z, qb, h[7]); h[8] = fma(z, qc, h[8]);
-                    }
+            for (int base = 0; base < na; base += 8 * kLiteBatch) {
+                Batch B;
+                load(base, !done, B);
+#pragma unroll
+                for (int k = 0; k < kLiteBatch; k++) {
+                    const bool sel = B.in[k] &&
+                        dist2_moved_fma(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]) <= lim;
+                    const double x = sel ? B.x[k] : 0.0, y = sel ? B.y[k] : 0.0, z = sel ? B.z[k] : 0.0;
+                    const double qa = sel ? B.a[k] : 0.0, qb = sel ? B.b[k] : 0.0, qc = sel ? B.c[k] : 0.0;
+                    cnt += sel ? 1 : 0;
+                    c[0] += x; c[1] += y; c[2] += z; c[3] += qa; c[4] += qb; c[5] += qc;
+                    h[0] = fma(x, qa, h[0]); h[1] = fma(x, qb, h[1]); h[2] = fma(x, qc, h[2]);
+                    h[3] = fma(y, qa, h[3]); h[4] = fma(y, qb, h[4]); h[5] = fma(y, qc, h[5]);
+                    h[6] = fma(z, qa, h[6]); h[7] = fma(z, qb, h[7]); h[8] = fma(z, qc, h[8]);
                 }
             }
             cnt = group8_isum(cnt);
@@ -443,15 +459,16 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
             }
             continue;
         }
-        if (!done) {
-            for (int q = s; q < na; q += 8) {
-                const int ap = L.alP[q], an = L.alN[q];
-                const double x = A.pcx[off + ap], y = A.pcy[off + ap], z = A.pcz[off + ap];
-                const double qa = A.qcx[off + an], qb = A.qcy[off + an], qc = A.qcz[off + an];
-                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
-                    cnt++;
-                    c[0] += x; c[1] += y; c[2] += z; c[3] += qa; c[4] += qb; c[5] += qc;
-                }
+        for (int base = 0; base < na; base += 8 * kLiteBatch) {
+            Batch B;
+            load(base, !done, B);
+#pragma unroll
+            for (int k = 0; k < kLiteBatch; k++) {
+                const bool sel = B.in[k] &&
+                    dist2_moved(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]) <= lim;
+                cnt += sel ? 1 : 0;
+                c[0] += sel ? B.x[k] : 0.0; c[1] += sel ? B.y[k] : 0.0; c[2] += sel ? B.z[k] : 0.0;
+                c[3] += sel ? B.a[k] : 0.0; c[4] += sel ? B.b[k] : 0.0; c[5] += sel ? B.c[k] : 0.0;
             }
         }
         cnt = group8_isum(cnt);
@@ -463,14 +480,18 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
         const double fc = (double)cnt;
         cp[0] = c[0] / fc; cp[1] = c[1] / fc; cp[2] = c[2] / fc;
         cq[0] = c[3] / fc; cq[1] = c[4] / fc; cq[2] = c[5] / fc;
-        if (solve) {
-            for (int q = s; q < na; q += 8) {
-                const int ap = L.alP[q], an = L.alN[q];
-                const double x = A.pcx[off + ap], y = A.pcy[off + ap], z = A.pcz[off + ap];
-                const double qa = A.qcx[off + an], qb = A.qcy[off + an], qc = A.qcz[off + an];
-                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
-                    const double d0 = x - cp[0], d1 = y - cp[1], d2 = z - cp[2];
-                    const double e0 = qa - cq[0], e1 = qb - cq[1], e2 = qc - cq[2];
+        if (__any(solve)) {
+            for (int base = 0; base < na; base += 8 * kLiteBatch) {
+                Batch B;
+                load(base, solve, B);
+#pragma unroll
+                for (int k = 0; k < kLiteBatch; k++) {
+                    const bool sel = B.in[k] &&
+                        dist2_moved(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]) <= lim;
+                    const double d0 = sel ? B.x[k] - cp[0] : 0.0, d1 = sel ? B.y[k] - cp[1] : 0.0;
+                    const double d2 = sel ? B.z[k] - cp[2] : 0.0;
+                    const double e0 = sel ? B.a[k] - cq[0] : 0.0, e1 = sel ? B.b[k] - cq[1] : 0.0;
+                    const double e2 = sel ? B.c[k] - cq[2] : 0.0;
                     h[0] += d0 * e0; h[1] += d0 * e1; h[2] += d0 * e2;
                     h[3] += d1 * e0; h[4] += d1 * e1; h[5] += d1 * e2;
                     h[6] += d2 * e0; h[7] += d2 * e1; h[8] += d2 * e2;
@@ -488,15 +509,20 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
             last = cnt;
         }
     }
-    // score with the d8 cut over the aligned set
+    // score with the d8 cut over the aligned set; a kept term has
+    // 1 + d2 / d0^2 <= 1 + sd8^2 / d0^2, far inside the exact reciprocal's range
     const double d0sq = fmax(pi.d0 * pi.d0, 1e-12), c8 = sd8 * sd8;
+    const double inv_d0sq = 1.0 / d0sq;
     double acc = 0.0;
-    if (gact) {
-        for (int q = s; q < na; q += 8) {
-            const int ap = L.alP[q], an = L.alN[q];
-            const double d2 = dist2_moved(R, t, A.pcx[off + ap], A.pcy[off + ap], A.pcz[off + ap],
-                                          A.qcx[off + an], A.qcy[off + an], A.qcz[off + an]);
-            if (d2 <= c8) acc += tm_term(d2, d0sq);
+    for (int base = 0; base < na; base += 8 * kLiteBatch) {
+        Batch B;
+        load(base, gact, B);
+#pragma unroll
+        for (int k = 0; k < kLiteBatch; k++) {
+            const double d2 = dist2_moved(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]);
+            const bool sel = B.in[k] && d2 <= c8;
+            const double tm = recip_in_range(1.0 + div_by(d2, d0sq, inv_d0sq));
+            acc += sel ? tm : 0.0;
         }
     }
     acc = group8_sum(acc);

@@ -11,17 +11,38 @@ IA_D double wave_sum(double v) {
     return v;
 }
 
+// Lane-xor partners inside groups of 8 lanes by DPP (VALU moves, no LDS
+// round trip): xor 1 and xor 2 are quad permutes; xor 4 takes lane + 4 in
+// banks 0 and 2 of each 16-lane row (row_shl:4) and lane - 4 in banks 1 and
+// 3 (row_shr:4).
+template <int M>
+IA_D int xor_lane(int v) {
+    static_assert(M == 1 || M == 2 || M == 4, "group-of-8 partner");
+    if constexpr (M == 1) return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false);
+    else if constexpr (M == 2) return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false);
+    else {
+        const int t = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xf, 0x5, false);
+        return __builtin_amdgcn_update_dpp(t, v, 0x114, 0xf, 0xA, false);
+    }
+}
+template <int M>
+IA_D double xor_lane(double v) {
+    return __hiloint2double(xor_lane<M>(__double2hiint(v)), xor_lane<M>(__double2loint(v)));
+}
+
+// Sum over each group of 8 lanes, left in every lane of the group (the
+// xor-1, 2, 4 butterfly; a + b and b + a round alike, so the 8 lanes agree).
 IA_D double group8_sum(double v) {
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
+    v += xor_lane<1>(v);
+    v += xor_lane<2>(v);
+    v += xor_lane<4>(v);
     return v;
 }
 
 IA_D int group8_isum(int v) {
-    v += __shfl_xor(v, 1, 64);
-    v += __shfl_xor(v, 2, 64);
-    v += __shfl_xor(v, 4, 64);
+    v += xor_lane<1>(v);
+    v += xor_lane<2>(v);
+    v += xor_lane<4>(v);
     return v;
 }
 
