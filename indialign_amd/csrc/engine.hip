@@ -549,8 +549,8 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     const ChainGroup *dg = upload(b_groups, groups);
     int *q = b_queue.get<int>(1);
     IA_CHECK(hipMemsetAsync(q, 0, sizeof(int), s_));
-    const int wpr = chain_trace_row_words(mn);
-    const size_t trace_bytes = (size_t)mp * wpr * 4;
+    const long long trace_w = chain_trace_words(mp, mn);
+    const size_t trace_bytes = (size_t)trace_w * 4;
     // LDS layout: the trace and the staged columns each go to LDS only when
     // that costs no resident wavefront (the fill is latency-bound, so
     // occupancy beats the global slots' extra traffic); otherwise the trace
@@ -572,7 +572,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     unsigned *trace = nullptr;
     long long slot_words = 0;
     if (!in_lds) {
-        slot_words = (long long)mp * wpr;
+        slot_words = trace_w;
         const long long budget = 8LL << 30;  // bytes of global trace slots
         slots = std::max(1LL, std::min(slots, budget / (slot_words * 4)));
         trace = b_trace.get<unsigned>((size_t)(slots * slot_words));

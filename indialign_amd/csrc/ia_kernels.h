@@ -208,8 +208,8 @@ void launch_gather_rows(hipStream_t s, const float *src, const float *tsrc, cons
                         int n, float *dst, float *tdst);
 
 size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds);
-// 32-bit trace words per compact row of a chain with nlen native columns
-int chain_trace_row_words(int nlen);
+// 32-bit trace words of the largest chain fill (max_plen x max_nlen)
+long long chain_trace_words(int max_plen, int max_nlen);
 int chain_blocks_per_cu(size_t lds, bool fast);
 
 }  // namespace ia

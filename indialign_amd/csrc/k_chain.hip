@@ -86,14 +86,22 @@ IA_D float vmax(float a, float b) {
 // the rest (phase B) need no per-lane row switch.
 //
 // Trace codes (0 diag, 1 up, 2 left) are packed by step: lane l shifts each
-// code into its word, and every 16 steps the wavefront stores one word per
-// lane to row a's slot (s >> 4) - (s(a, 0) >> 4) (a word that spans a row
-// switch goes to both rows).  H = max(diag, up, left) is exact; ties only
+// code into its word, and every 16 steps the wavefront stores its 64 words
+// side by side (word (s >> 4) * 64 + l), one coalesced 256-byte store; the
+// traceback finds cell (a, b) from its step s(a, b) alone.  H = max(diag, up, left) is exact; ties only
 // decide the trace code; G = H + gap_open for diagonally entered cells, else
 // H, is what down and right neighbours add.
 IA_HD int fill_period(int nlen) { return nlen > 65 ? nlen : 65; }
-IA_HD int trace_row_words(int nlen) { return ((nlen + 14) >> 4) + 1; }
-IA_HD long long fill_row_start(int a, int P) { return (long long)(a >> 6) * P + (a & 63); }
+IA_HD int fill_row_start(int a, int P) { return (a >> 6) * P + (a & 63); }
+// steps of a plen x nlen fill, and its trace words (16 steps x 64 lanes per
+// 64 words; word (s >> 4) * 64 + l holds lane l's codes of steps s & ~15 ..)
+IA_HD int fill_steps(int plen, int nlen) {
+    const int M = (plen + 63) >> 6;
+    return (M - 1) * fill_period(nlen) + (plen - 1 - 64 * (M - 1)) + nlen;
+}
+IA_HD long long trace_words(int plen, int nlen) {
+    return (long long)((fill_steps(plen, nlen) + 15) >> 4) * 64;
+}
 
 template <int SMODE, class TR, bool CL, class V>
 IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const double *t,
@@ -102,7 +110,6 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     const int plen = pi.plen, nlen = pi.nlen, off = pi.off;
     constexpr bool F32 = sizeof(V) == 4;
     const int P = fill_period(nlen);
-    const int wpr = trace_row_words(nlen);
     const int M = (plen + 63) >> 6;
     const int lastl = plen - 1 - 64 * (M - 1);
     // column operands: the wavefront's LDS copy (FP64, or FP32 for the fast
@@ -138,8 +145,6 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     char sp = 0;
     int a = lane - 64, b = P - lane;        // current row and column (lane l
                                             // reaches b = P, its first row, at step l)
-    int start = 0, a_old = -1, start_old = 0;  // first step of the current / previous row
-    bool sww = false;                       // switched rows within the current trace word
     V H = 0, G = 0, D = 0;
     unsigned tw = 0;
     // column operands of the current step.  FP64: loaded per lane one step
@@ -158,14 +163,9 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     V rH = 0, rG = 0;
     int st = 0;
 
+    // store the word of steps (st & ~15) .. st, the last nsteps codes
     auto flush = [&](int nsteps) {
-        const unsigned w = nsteps == 16 ? tw : tw >> (32 - 2 * nsteps);
-        const int w0 = st - nsteps + 1;  // first step of the word
-        if ((unsigned)a < (unsigned)plen && w0 < start + nlen)
-            trace[(long long)a * wpr + ((st >> 4) - (start >> 4))] = w;
-        if (sww && (unsigned)a_old < (unsigned)plen && w0 < start_old + nlen)
-            trace[(long long)a_old * wpr + ((st >> 4) - (start_old >> 4))] = w;
-        sww = false;
+        trace[(long long)(st >> 4) * 64 + lane] = nsteps == 16 ? tw : tw >> (32 - 2 * nsteps);
     };
 
     // PA: phase A (lane q starts its next row).  GC: the general next-column
@@ -188,8 +188,7 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         if constexpr (PA) {
             const bool e = lane == q;
             if (e) {
-                a_old = a; start_old = start; sww = true;
-                a += 64; b = 0; start = st;
+                a += 64; b = 0;
                 mx = nmx; my = nmy; mz = nmz; sp = nsp;
             }
             Dc = e ? (V)0 : D;
@@ -278,10 +277,10 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
 }
 
 // Trace word of cell (a, b) and the code's bit position in it.
-IA_D long long trace_word_index(int a, int b, int P, int wpr, int &shift) {
-    const long long s0 = fill_row_start(a, P), s = s0 + b;
-    shift = (int)(s & 15) * 2;
-    return (long long)a * wpr + ((s >> 4) - (s0 >> 4));
+IA_D int trace_word_index(int a, int b, int P, int &shift) {
+    const int s = fill_row_start(a, P) + b;
+    shift = (s & 15) * 2;
+    return (s >> 4) * 64 + (a & 63);
 }
 
 // Needleman-Wunsch fill + traceback; returns the aligned count (pairs in
@@ -292,7 +291,6 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
     const int lane = threadIdx.x & 63;
     const int plen = pi.plen, nlen = pi.nlen;
     const int P = fill_period(nlen);
-    const int wpr = trace_row_words(nlen);
     typedef typename std::conditional<FAST, float, double>::type V;
     nw_fill<SMODE, TR, CL, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
@@ -304,7 +302,7 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
             int i = plen, j = nlen, k = 0;
             while (i > 0 && j > 0) {
                 int sh;
-                const unsigned w = trace[trace_word_index(i - 1, j - 1, P, wpr, sh)];
+                const unsigned w = trace[trace_word_index(i - 1, j - 1, P, sh)];
                 const unsigned d = (w >> sh) & 3u;
                 if (d == 0) {
                     L.alP[k] = (short)(i - 1);
@@ -321,34 +319,29 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
             L.misc[0] = k;
         }
     } else {
-        // trace in global memory: one round of loads brings a window of 64
-        // rows (lane l holds row top-1-l) by two words (the word holding the
-        // current column and the one before it) into registers; the
-        // wavefront walks it with wave-uniform (i, j) and fetches each step's
-        // word by readlane, reloading when the path leaves the window.
+        // trace in global memory: a window of four step-words (lane q holds
+        // lane q's words of steps 16 (w0 - k) .. 16 (w0 - k) + 15, k < 4) is
+        // loaded with four coalesced loads; the wavefront walks it with
+        // wave-uniform (i, j), reading each cell's word by readlane, and
+        // reloads when the path's step leaves the window (steps only
+        // decrease along the path).
         int i = plen, j = nlen, k = 0;
         while (i > 0 && j > 0) {
-            const int top = i;
-            const int r = top - 1 - lane;
-            unsigned w1 = 0, w0 = 0;
-            int kk = 0;
-            if (r >= 0) {
-                const long long s0 = fill_row_start(r, P);
-                kk = (int)(((s0 + j - 1) >> 4) - (s0 >> 4));
-                w1 = trace[(long long)r * wpr + kk];
-                if (kk > 0) w0 = trace[(long long)r * wpr + kk - 1];
-            }
+            const int w0 = (fill_row_start(i - 1, P) + j - 1) >> 4;
+            unsigned wv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) wv[q] = w0 - q >= 0 ? trace[(long long)(w0 - q) * 64 + lane] : 0u;
             while (i > 0 && j > 0) {
-                const int rl = top - i;
-                if (rl >= 64) break;
-                const long long s0 = fill_row_start(i - 1, P), s = s0 + j - 1;
-                const int kw = (int)((s >> 4) - (s0 >> 4));
-                const int kr = __builtin_amdgcn_readlane(kk, rl);
-                if (kw != kr && kw != kr - 1) break;
-                const unsigned a1 = (unsigned)__builtin_amdgcn_readlane((int)w1, rl);
-                const unsigned a0 = (unsigned)__builtin_amdgcn_readlane((int)w0, rl);
-                const unsigned w = kw == kr ? a1 : a0;
-                const unsigned d = (w >> ((int)(s & 15) * 2)) & 3u;
+                const int s = fill_row_start(i - 1, P) + j - 1;
+                const int kw = w0 - (s >> 4);
+                if (kw > 3) break;
+                const int l = (i - 1) & 63;
+                const unsigned a0 = (unsigned)__builtin_amdgcn_readlane((int)wv[0], l);
+                const unsigned a1 = (unsigned)__builtin_amdgcn_readlane((int)wv[1], l);
+                const unsigned a2 = (unsigned)__builtin_amdgcn_readlane((int)wv[2], l);
+                const unsigned a3 = (unsigned)__builtin_amdgcn_readlane((int)wv[3], l);
+                const unsigned w = kw == 0 ? a0 : kw == 1 ? a1 : kw == 2 ? a2 : a3;
+                const unsigned d = (w >> ((s & 15) * 2)) & 3u;
                 if (d == 0) {
                     if (lane == 0) {
                         L.alP[k] = (short)(i - 1);
@@ -784,7 +777,7 @@ struct LdsPlan {
         misc = o; o += 16;
         cols = o; o += (nc + 15) & ~(size_t)15;
         trace = o;
-        if (trace_in_lds) o += (size_t)max_plen * (size_t)trace_row_words(max_nlen) * 4;
+        if (trace_in_lds) o += (size_t)trace_words(max_plen, max_nlen) * 4;
         total = o;
     }
 };
@@ -839,7 +832,7 @@ int chain_blocks_per_cu(size_t lds, bool fast) {
     return e == hipSuccess ? nb : 1;
 }
 
-int chain_trace_row_words(int nlen) { return trace_row_words(nlen); }
+long long chain_trace_words(int max_plen, int max_nlen) { return trace_words(max_plen, max_nlen); }
 
 size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds) {
     return LdsPlan(max_plen, max_nlen, trace_in_lds, cols_in_lds).total;
