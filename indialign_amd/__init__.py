@@ -93,6 +93,10 @@ def load_library(path: str | None = None):
     lib.indialign_set_learned_budgets.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                   ctypes.c_int, ctypes.c_int, ctypes.c_double]
     lib.indialign_set_learned_budgets.restype = ctypes.c_int
+    lib.indialign_set_launch_tuning.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.indialign_set_launch_tuning.restype = ctypes.c_int
+    lib.indialign_get_launch_tuning.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    lib.indialign_get_launch_tuning.restype = ctypes.c_int
     lib.indialign_collect_records.argtypes = [
         ctypes.c_int, ctypes.POINTER(NativePairInput), ctypes.POINTER(NativeConfig),
         ctypes.POINTER(NativePairResult), c_float_p, c_float_p, ctypes.POINTER(ctypes.c_int),
@@ -283,6 +287,26 @@ def set_ranker(params, top_m: int = 64, fallback_margin: float = -1.0,
     rc = lib.indialign_set_ranker(p.ctypes.data_as(_f32p))
     if rc != 0:
         raise IndialignError(f"indialign_set_ranker returned {rc}")
+
+
+def set_launch_tuning(chain_waves: int = 3, chain_slots_per_cu: int = 0,
+                      chain_trace: int = 0) -> None:
+    """Launch shape of the NW chain kernel (indialign_set_launch_tuning):
+    wavefronts per SIMD of the kernel variant (2 or 3), persistent slots per
+    CU (0 = all resident), trace placement (0 auto, 1 LDS when it fits,
+    2 global).  Results do not depend on it."""
+    rc = load_library().indialign_set_launch_tuning(int(chain_waves), int(chain_slots_per_cu),
+                                                    int(chain_trace))
+    if rc != 0:
+        raise IndialignError(f"indialign_set_launch_tuning({chain_waves}, {chain_slots_per_cu}, "
+                             f"{chain_trace}) returned {rc}")
+
+
+def get_launch_tuning() -> tuple[int, int, int]:
+    """Current (chain_waves, chain_slots_per_cu, chain_trace)."""
+    out = (ctypes.c_int * 3)()
+    load_library().indialign_get_launch_tuning(out)
+    return int(out[0]), int(out[1]), int(out[2])
 
 
 def collect_records(pairs, cap: int, cfg: NativeConfig | None = None):

@@ -11,6 +11,8 @@
 #include "indialign.h"
 #include "mlp.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -136,6 +138,13 @@ class Engine {
     int top_offsets = 16;           // threading-DP offsets per pair that start chains
     int top_offset_banks = 32;      // threading-DP offsets per pair that run their bank
     int top_seeds = 0;              // seeds refined per SS-stage and threading bank (0 = all)
+    // Chain-kernel launch shape (indialign_set_launch_tuning): wavefronts per
+    // SIMD of the kernel variant (2 or 3), persistent slots per CU (0: as many
+    // as are resident), NW trace placement (0: LDS only when that costs no
+    // resident wavefront, 1: LDS whenever it fits, 2: always a global slot).
+    int chain_waves = 3;
+    int chain_slots_per_cu = 0;
+    int chain_trace = 0;
     // Record sink: when set, every local-fragment work item of the search
     // appends its descriptor row, its refined score (best of its two
     // chains) and its pair index, up to cap rows.
@@ -559,15 +568,19 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     constexpr size_t kLdsMax = 160 * 1024;
     auto occupancy = [&](bool tr, bool cl) {
         const size_t b = chain_lds_bytes(mp, mn, tr, cl);
-        return b > kLdsMax ? 0 : chain_blocks_per_cu(b, fast_);
+        return b > kLdsMax ? 0 : chain_blocks_per_cu(b, fast_, chain_waves);
     };
     const int occ_free = occupancy(false, false);
     if (occ_free <= 0) throw DevError("chain too long for one workgroup's LDS");
-    const bool in_lds = trace_bytes <= 96 * 1024 && occupancy(true, false) >= occ_free;
+    const bool lds_fits = trace_bytes <= 96 * 1024 && occupancy(true, false) > 0;
+    const bool in_lds = chain_trace == 2 ? false
+                        : chain_trace == 1 ? lds_fits
+                                           : lds_fits && occupancy(true, false) >= occ_free;
     const bool cols_lds = occupancy(in_lds, true) >= occupancy(in_lds, false);
     const size_t lds = chain_lds_bytes(mp, mn, in_lds, cols_lds);
-    int per_cu = std::max(1, chain_blocks_per_cu(lds, fast_));
+    int per_cu = std::max(1, chain_blocks_per_cu(lds, fast_, chain_waves));
     per_cu = std::min(per_cu, 16);
+    if (chain_slots_per_cu > 0) per_cu = std::min(per_cu, chain_slots_per_cu);
     long long slots = (long long)n_cu_ * per_cu;
     unsigned *trace = nullptr;
     long long slot_words = 0;
@@ -582,7 +595,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     launch_chains(s_, A_, dg, (int)groups.size(), total, xf_dev, out_dev,
                   b_lout.get<ListOut>(1), b_lists_p.get<int>(1), b_lists_n.get<int>(1),
                   b_list_na.get<int>(1), list_cap_, mp, mn, q, trace, slot_words, (int)slots,
-                  cols_lds, fast_);
+                  cols_lds, fast_, chain_waves);
     launched();
     IA_CHECK(hipGetLastError());
     IA_CHECK(hipEventRecord(e1_, s_));
@@ -1455,6 +1468,65 @@ int g_top_seeds = 0;
 unsigned g_ranker_ver = 0;
 unsigned g_engine_ranker_ver = 0;
 
+// Chain-kernel launch shape: {waves per SIMD, slots per CU, trace placement}
+// (Engine::chain_*).  Read once from a tuning file written by the autotuner,
+// then settable with indialign_set_launch_tuning.
+int g_tune[3] = {3, 0, 0};
+bool g_tune_loaded = false;
+
+bool tune_valid(int waves, int slots, int trace) {
+    return (waves == 2 || waves == 3) && slots >= 0 && slots <= 64 && trace >= 0 && trace <= 2;
+}
+
+// The tuning file is $INDIALIGN_TUNING, or indialign_tuning.txt beside this
+// library; lines "chain_waves 3", "chain_slots_per_cu 0", "chain_trace 0"
+// ('#' starts a comment).  A missing file keeps the defaults; a malformed
+// one is reported and ignored.
+void load_tuning_file() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    std::string path;
+    const char *env = std::getenv("INDIALIGN_TUNING");
+    if (env && *env) {
+        path = env;
+    } else {
+        Dl_info info;
+        if (!dladdr((void *)&load_tuning_file, &info) || !info.dli_fname) return;
+        path = info.dli_fname;
+        const size_t cut = path.find_last_of('/');
+        path = (cut == std::string::npos ? std::string(".") : path.substr(0, cut)) +
+               "/indialign_tuning.txt";
+    }
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return;
+    int t[3] = {g_tune[0], g_tune[1], g_tune[2]};
+    char line[256];
+    bool ok = true;
+    while (std::fgets(line, sizeof line, f)) {
+        char key[64];
+        int v = 0;
+        if (line[0] == '#' || std::sscanf(line, "%63s", key) != 1) continue;
+        if (std::sscanf(line, "%63s %d", key, &v) != 2) { ok = false; break; }
+        if (!std::strcmp(key, "chain_waves")) t[0] = v;
+        else if (!std::strcmp(key, "chain_slots_per_cu")) t[1] = v;
+        else if (!std::strcmp(key, "chain_trace")) t[2] = v;
+    }
+    std::fclose(f);
+    if (ok && tune_valid(t[0], t[1], t[2])) {
+        std::memcpy(g_tune, t, sizeof t);
+    } else {
+        std::fprintf(stderr, "indialign_amd: ignoring malformed tuning file %s\n", path.c_str());
+    }
+}
+
+// With g_mu held and g_engine alive: install the current launch shape.
+void sync_tuning() {
+    load_tuning_file();
+    g_engine->chain_waves = g_tune[0];
+    g_engine->chain_slots_per_cu = g_tune[1];
+    g_engine->chain_trace = g_tune[2];
+}
+
 // With g_mu held and g_engine alive: install the current ranker.
 void sync_ranker() {
     static ia::Engine *seen = nullptr;
@@ -1508,6 +1580,7 @@ int run_search(int n, const NativePairInput *inp, const NativeConfig *cfg, Nativ
             throw ia::DevError("no HIP device available");
         if (!g_engine) g_engine.reset(new ia::Engine());
         sync_ranker();
+        sync_tuning();
         g_engine->load(inp, n);
         const int rc = g_engine->search(cfg, serial != 0, out, method);
         std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
@@ -1577,6 +1650,7 @@ int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int 
     if (method == 2 && g_ranker.empty()) return -4;
     try {
         sync_ranker();
+        sync_tuning();
         const int rc = g_engine->search(cfg, serial_bank != 0, outputs, method);
         std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
         return rc;
@@ -1642,6 +1716,23 @@ int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets,
     return 0;
 }
 
+int indialign_set_launch_tuning(int chain_waves, int chain_slots_per_cu, int chain_trace) {
+    if (!tune_valid(chain_waves, chain_slots_per_cu, chain_trace)) return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_tune_loaded = true;  // an explicit setting wins over the file
+    g_tune[0] = chain_waves;
+    g_tune[1] = chain_slots_per_cu;
+    g_tune[2] = chain_trace;
+    return 0;
+}
+
+int indialign_get_launch_tuning(int *out3) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    load_tuning_file();
+    for (int k = 0; k < 3; k++) out3[k] = g_tune[k];
+    return 0;
+}
+
 int indialign_collect_records(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
                               NativePairResult *outputs, float *X, float *y, int *pair_idx,
                               long long cap, long long *n_records) {
@@ -1652,6 +1743,7 @@ int indialign_collect_records(int n_pairs, const NativePairInput *inputs, const 
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
             throw ia::DevError("no HIP device available");
         if (!g_engine) g_engine.reset(new ia::Engine());
+        sync_tuning();
         ia::Engine::RecordSink sink{X, y, pair_idx, cap, 0};
         g_engine->rec = &sink;
         g_engine->load(inputs, n_pairs);

@@ -119,7 +119,7 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
-                   int n_slots, bool cols_in_lds, bool fast);
+                   int n_slots, bool cols_in_lds, bool fast, int waves);
 // ---- learned ranking of local-fragment work items ------------------------
 constexpr int kFeat = 96;   // descriptor width
 constexpr int kHid = 256;   // hidden units per layer
@@ -210,6 +210,7 @@ void launch_gather_rows(hipStream_t s, const float *src, const float *tsrc, cons
 size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds);
 // 32-bit trace words of the largest chain fill (max_plen x max_nlen)
 long long chain_trace_words(int max_plen, int max_nlen);
-int chain_blocks_per_cu(size_t lds, bool fast);
+// resident chain wavefronts per CU for the W = waves variant
+int chain_blocks_per_cu(size_t lds, bool fast, int waves);
 
 }  // namespace ia

@@ -782,11 +782,11 @@ struct LdsPlan {
     }
 };
 
-// At most 168 VGPRs: three wavefronts per SIMD hide the dependency latency of
-// the fill (measured 1.37x faster than one wavefront per SIMD on 40-300 nt
-// chains; the spills this costs sit outside the fill loop).
-template <bool FAST>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void chain_kernel(Arena A, const ChainGroup *groups,
+// W wavefronts per SIMD: W = 3 caps the kernel at 168 VGPRs (three resident
+// wavefronts hide each other's latency; the spills this costs sit outside
+// the fill loop), W = 2 at 256.  Which one runs is a launch-tuning choice.
+template <bool FAST, int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void chain_kernel(Arena A, const ChainGroup *groups,
                                                    int n_groups, long long n_chains,
                                                    const Hit *xf, Hit *out, ListOut *lout,
                                                    int *lists_p, int *lists_n, int *list_na,
@@ -824,11 +824,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void ch
 
 }  // namespace
 
-int chain_blocks_per_cu(size_t lds, bool fast) {
+int chain_blocks_per_cu(size_t lds, bool fast, int waves) {
     int nb = 0;
-    const hipError_t e =
-        fast ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<true>, 64, lds)
-             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<false>, 64, lds);
+    hipError_t e;
+    if (waves == 2)
+        e = fast ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<true, 2>, 64, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<false, 2>, 64, lds);
+    else
+        e = fast ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<true, 3>, 64, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<false, 3>, 64, lds);
     return e == hipSuccess ? nb : 1;
 }
 
@@ -842,18 +846,22 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
-                   int n_slots, bool cols_in_lds, bool fast) {
+                   int n_slots, bool cols_in_lds, bool fast, int waves) {
     if (n_chains <= 0) return;
     const size_t lds = chain_lds_bytes(max_plen, max_nlen, trace_scratch == nullptr, cols_in_lds);
     const int grid = n_slots;
-    if (fast)
-        hipLaunchKernelGGL(chain_kernel<true>, dim3(grid), dim3(64), lds, s, A, groups, n_groups,
-                           n_chains, xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen,
-                           max_nlen, queue, trace_scratch, trace_slot_words, (int)cols_in_lds);
-    else
-        hipLaunchKernelGGL(chain_kernel<false>, dim3(grid), dim3(64), lds, s, A, groups, n_groups,
-                           n_chains, xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen,
-                           max_nlen, queue, trace_scratch, trace_slot_words, (int)cols_in_lds);
+#define IA_LAUNCH_CHAIN(F, W)                                                                    \
+    hipLaunchKernelGGL((chain_kernel<F, W>), dim3(grid), dim3(64), lds, s, A, groups, n_groups, \
+                       n_chains, xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen,   \
+                       max_nlen, queue, trace_scratch, trace_slot_words, (int)cols_in_lds)
+    if (waves == 2) {
+        if (fast) IA_LAUNCH_CHAIN(true, 2);
+        else IA_LAUNCH_CHAIN(false, 2);
+    } else {
+        if (fast) IA_LAUNCH_CHAIN(true, 3);
+        else IA_LAUNCH_CHAIN(false, 3);
+    }
+#undef IA_LAUNCH_CHAIN
 }
 
 }  // namespace ia

@@ -1,0 +1,89 @@
+"""Launch tuning of the chain kernel (indialign_set_launch_tuning and the
+tuning file the autotuner writes): range checks and file parsing run on the
+CPU (the library reads the file before touching a device); on the GPU every
+shape must give the same results, bit for bit."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import indialign_amd as ia
+from indialign_amd import autotune, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PROBE = r"""
+import sys
+sys.path.insert(0, %r)
+import indialign_amd as ia
+print(list(ia.get_launch_tuning()))
+"""
+
+
+def read_tuning_in_subprocess(tuning_path):
+    env = dict(os.environ, INDIALIGN_TUNING=tuning_path)
+    r = subprocess.run([sys.executable, "-c", PROBE % ROOT], capture_output=True, text=True, env=env)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout.strip().splitlines()[-1]), r.stderr
+
+
+def test_setter_range_checks():
+    before = ia.get_launch_tuning()
+    for bad in [(1, 0, 0), (4, 0, 0), (3, -1, 0), (3, 65, 0), (3, 0, 3), (2, 0, -1)]:
+        with pytest.raises(ia.IndialignError):
+            ia.set_launch_tuning(*bad)
+        assert ia.get_launch_tuning() == before
+    ia.set_launch_tuning(2, 8, 2)
+    assert ia.get_launch_tuning() == (2, 8, 2)
+    ia.set_launch_tuning(*before)
+
+
+def test_tuning_file_read_by_library(tmp_path):
+    p = tmp_path / "t.txt"
+    autotune.write_tuning(str(p), (2, 4, 1), "test")
+    got, _ = read_tuning_in_subprocess(str(p))
+    assert got == [2, 4, 1]
+    # malformed or out-of-range files are reported and ignored
+    p.write_text("chain_waves 7\n")
+    got, err = read_tuning_in_subprocess(str(p))
+    assert got == [3, 0, 0] and "malformed" in err
+    p.write_text("chain_waves\n")
+    got, err = read_tuning_in_subprocess(str(p))
+    assert got == [3, 0, 0] and "malformed" in err
+    # a missing file keeps the defaults
+    got, _ = read_tuning_in_subprocess(str(tmp_path / "absent.txt"))
+    assert got == [3, 0, 0]
+
+
+def test_candidate_shapes():
+    c = autotune.candidates()
+    assert c[0] == (3, 0, 0) and len(set(c)) == len(c)
+    for w, s, t in c:
+        assert w in (2, 3) and 0 <= s <= 64 and t in (0, 1, 2)
+
+
+@pytest.mark.gpu
+def test_launch_shapes_give_identical_results():
+    """Short pairs plus one long masked pair, plain and fast methods: every
+    launch shape returns the same scores, rotations and translations."""
+    if not ia.gpu_available():
+        pytest.fail("no GPU visible to the HIP runtime")
+    before = ia.get_launch_tuning()
+    pairs = synth.short_pairs(24) + synth.long_pairs(1)
+    try:
+        for method in (0, 1):
+            ref = None
+            for shape in [(3, 0, 0), (2, 0, 0), (3, 4, 2), (2, 8, 1), (3, 0, 1)]:
+                ia.set_launch_tuning(*shape)
+                out = ia.tmscore_search_batch(pairs, method=method)
+                print(f"method {method} shape {shape}: mean score {out[0].mean():.12f}", flush=True)
+                if ref is None:
+                    ref = out
+                    continue
+                for a, b in zip(out, ref):
+                    assert np.array_equal(a, b), (method, shape)
+    finally:
+        ia.set_launch_tuning(*before)
