@@ -105,6 +105,24 @@ def test_long_pairs_global_trace_vs_oracle():
     assert ref[-1, 12] < 0.3  # the dense rescue pass ran
 
 
+def test_fill_round_boundaries_vs_oracle():
+    """Chain sizes at the NW fill's round boundaries (a lane computes rows
+    l, l+64, ...; the period is max(nlen, 65)): N = 63-66, 127-130, 192, 193,
+    and masked pairs whose compact row and column counts differ.  The noise
+    keeps every pair below the rescue thresholds, so the dense rescue runs
+    its chains at exactly these sizes."""
+    _gpu_or_skip()
+    pairs = [synth.make_pair(30000 + n, n=n, sigma=10.0)
+             for n in (63, 64, 65, 66, 127, 128, 129, 130, 192, 193)]
+    pairs += [synth.make_pair(30500 + n, n=n, sigma=9.0, drop_frac=0.15) for n in (70, 140, 200)]
+    s, R, t = ia.tmscore_search_batch(pairs)
+    ref = libs.oracle_search(pairs, serial_bank=1)
+    _check(s, R, t, ref, "fill-boundaries")
+    assert (ref[:, 12] < 0.3).all()  # every pair ran the dense rescue
+    for p in pairs[-3:]:
+        assert p.pvalid.sum() != p.nvalid.sum() or p.pvalid.sum() != len(p.pvalid)
+
+
 def test_reference_toy_pair():
     """The 5-residue toy of the reference tests (tests/test_gpu_scoring_refactor.py:179-190):
     the reference C path scores it 0.77317584653223026 for dp_iter 0 and 1."""
