@@ -149,7 +149,7 @@ int indialign_rotation_from_covariance(const double *h, int method, double *R);
  * method 2.  Returns 0. */
 int indialign_set_ranker(const float *params);
 
-/* Work budgets of method 2 (defaults 64, 16, 16, 32, 0, -1):
+/* Work budgets of method 2 (defaults 64, 16, 16, 16, 0, -1):
  *   top_m >= 1: dense local-fragment work items refined per pair (rescue.h
  *     dense pass), the ranker's top ones;
  *   top_m_first >= 1: the same for the first local-fragment pass;

@@ -264,7 +264,7 @@ def _f32(a) -> np.ndarray:
 
 def set_ranker(params, top_m: int = 64, fallback_margin: float = -1.0,
                top_offsets: int = 16, top_seeds: int = 0, top_m_first: int = 16,
-               top_offset_banks: int = 32) -> None:
+               top_offset_banks: int = 16) -> None:
     """Install the ranker (NPARAM float32 values) used by METHOD_LEARNED, or
     remove it with params=None, and set the method's budgets
     (indialign_set_learned_budgets): top_m dense and top_m_first first-pass
@@ -390,7 +390,7 @@ DEFAULT_RANKER = os.path.join(HERE, "ranker_params.npy")
 
 def load_default_ranker(top_m: int = 64, fallback_margin: float = -1.0,
                         top_offsets: int = 16, top_seeds: int = 0, top_m_first: int = 16,
-                        top_offset_banks: int = 32, path: str | None = None) -> np.ndarray:
+                        top_offset_banks: int = 16, path: str | None = None) -> np.ndarray:
     """Install the packaged trained ranker (train_ranker.py) for METHOD_LEARNED."""
     p = np.load(path or DEFAULT_RANKER, allow_pickle=False)
     set_ranker(p, top_m, fallback_margin, top_offsets, top_seeds, top_m_first, top_offset_banks)

@@ -136,7 +136,7 @@ class Engine {
     int top_m_first = 16;           // first local-fragment pass
     double fallback_margin = -1.0;  // < 0: never fall back to every work item
     int top_offsets = 16;           // threading-DP offsets per pair that start chains
-    int top_offset_banks = 32;      // threading-DP offsets per pair that run their bank
+    int top_offset_banks = 16;      // threading-DP offsets per pair that run their bank
     int top_seeds = 0;              // seeds refined per SS-stage and threading bank (0 = all)
     // Chain-kernel launch shape (indialign_set_launch_tuning): wavefronts per
     // SIMD of the kernel variant (2 or 3), persistent slots per CU (0: as many
@@ -1463,7 +1463,7 @@ int g_top_m = 64;
 int g_top_m_first = 16;
 double g_margin = -1.0;
 int g_top_offsets = 16;
-int g_top_offset_banks = 32;
+int g_top_offset_banks = 16;
 int g_top_seeds = 0;
 unsigned g_ranker_ver = 0;
 unsigned g_engine_ranker_ver = 0;
