@@ -170,22 +170,27 @@ int indialign_set_ranker(const float *params);
 int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets,
                                   int top_offset_banks, int top_seeds, double fallback_margin);
 
-/* Launch shape of the NW chain kernel (no counterpart in the reference: its
- * CPU loops have no launch shape).  For callers and the project's autotuner
- * (indialign_amd/autotune.py), which picks the fastest shape on a sample of
- * the workload and writes it to indialign_tuning.txt beside the library (or
- * to the file named by INDIALIGN_TUNING), read at the first search.
- *   chain_waves 2 or 3: wavefronts per SIMD the kernel variant is compiled
- *     for (register budget 256 or 168 VGPRs);
- *   chain_slots_per_cu 0..64: persistent wavefronts per CU (0: as many as
- *     are resident);
+/* Launch shape of the NW chain kernel and the seed-bank kernel (no
+ * counterpart in the reference: its CPU loops have no launch shape).  For
+ * callers and the project's autotuner (indialign_amd/autotune.py), which
+ * picks the fastest shape on a sample of the workload and writes it to
+ * indialign_tuning.txt beside the library (or to the file named by
+ * INDIALIGN_TUNING), read at the first search.
+ *   chain_waves 2 or 3: wavefronts per SIMD the chain kernel variant is
+ *     compiled for (register budget 256 or 168 VGPRs);
+ *   chain_slots_per_cu 0..64: persistent chain wavefronts per CU (0: as many
+ *     as are resident);
  *   chain_trace 0, 1 or 2: NW trace in LDS only when that costs no resident
- *     wavefront / in LDS whenever it fits / always in a global slot.
+ *     wavefront / in LDS whenever it fits / always in a global slot;
+ *   bank_pass_cap 0..64: refinement passes a seed-bank candidate runs in the
+ *     bank launch; candidates still refining continue in a second launch
+ *     that packs them densely (0: a single launch).
  * Results do not depend on the shape.  Returns 0, or -1 for a value out of
  * range (the setting is then unchanged). */
-int indialign_set_launch_tuning(int chain_waves, int chain_slots_per_cu, int chain_trace);
-/* Current launch shape into out3[0..2] (file or defaults 3, 0, 0). */
-int indialign_get_launch_tuning(int *out3);
+int indialign_set_launch_tuning(int chain_waves, int chain_slots_per_cu, int chain_trace,
+                                int bank_pass_cap);
+/* Current launch shape into out4[0..3] (file or defaults 3, 0, 0, 5). */
+int indialign_get_launch_tuning(int *out4);
 
 /* Plain search (method 0, serial seed bank) that also records every
  * local-fragment work item: its descriptor row (X, INDIALIGN_MLP_NFEAT floats

@@ -93,7 +93,8 @@ def load_library(path: str | None = None):
     lib.indialign_set_learned_budgets.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                   ctypes.c_int, ctypes.c_int, ctypes.c_double]
     lib.indialign_set_learned_budgets.restype = ctypes.c_int
-    lib.indialign_set_launch_tuning.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.indialign_set_launch_tuning.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int]
     lib.indialign_set_launch_tuning.restype = ctypes.c_int
     lib.indialign_get_launch_tuning.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.indialign_get_launch_tuning.restype = ctypes.c_int
@@ -290,23 +291,25 @@ def set_ranker(params, top_m: int = 64, fallback_margin: float = -1.0,
 
 
 def set_launch_tuning(chain_waves: int = 3, chain_slots_per_cu: int = 0,
-                      chain_trace: int = 0) -> None:
-    """Launch shape of the NW chain kernel (indialign_set_launch_tuning):
-    wavefronts per SIMD of the kernel variant (2 or 3), persistent slots per
-    CU (0 = all resident), trace placement (0 auto, 1 LDS when it fits,
-    2 global).  Results do not depend on it."""
+                      chain_trace: int = 0, bank_pass_cap: int = 5) -> None:
+    """Launch shape (indialign_set_launch_tuning): the NW chain kernel's
+    wavefronts per SIMD (2 or 3), persistent slots per CU (0 = all
+    resident) and trace placement (0 auto, 1 LDS when it fits, 2 global),
+    and the seed banks' pass cap (0 = one launch; else candidates still
+    refining after that many passes continue in a compacted launch).
+    Results do not depend on it."""
     rc = load_library().indialign_set_launch_tuning(int(chain_waves), int(chain_slots_per_cu),
-                                                    int(chain_trace))
+                                                    int(chain_trace), int(bank_pass_cap))
     if rc != 0:
         raise IndialignError(f"indialign_set_launch_tuning({chain_waves}, {chain_slots_per_cu}, "
-                             f"{chain_trace}) returned {rc}")
+                             f"{chain_trace}, {bank_pass_cap}) returned {rc}")
 
 
-def get_launch_tuning() -> tuple[int, int, int]:
-    """Current (chain_waves, chain_slots_per_cu, chain_trace)."""
-    out = (ctypes.c_int * 3)()
+def get_launch_tuning() -> tuple[int, int, int, int]:
+    """Current (chain_waves, chain_slots_per_cu, chain_trace, bank_pass_cap)."""
+    out = (ctypes.c_int * 4)()
     load_library().indialign_get_launch_tuning(out)
-    return int(out[0]), int(out[1]), int(out[2])
+    return int(out[0]), int(out[1]), int(out[2]), int(out[3])
 
 
 def collect_records(pairs, cap: int, cfg: NativeConfig | None = None):

@@ -145,6 +145,9 @@ class Engine {
     int chain_waves = 3;
     int chain_slots_per_cu = 0;
     int chain_trace = 0;
+    // Seed banks: refinement passes a candidate runs in the first launch
+    // before it is left to the continuation launch (0: one launch).
+    int bank_pass_cap = 5;
     // Record sink: when set, every local-fragment work item of the search
     // appends its descriptor row, its refined score (best of its two
     // chains) and its pair index, up to cap rows.
@@ -213,7 +216,8 @@ class Engine {
         b_lists_n, b_list_na, b_lout, b_groups, b_xf, b_chain_out, b_queue, b_trace, b_red_off,
         b_red_cnt, b_red_out, b_rs_pair, b_rs_in, b_rs_out, b_pairs, b_aoff, b_acnt, b_ascr,
         b_soff, b_fofs, b_fcnt, b_moff, b_feat, b_fgroups, b_pred, b_isel, b_srow, b_sitems,
-        b_ssel, b_scnt, b_itemy, b_pmax;
+        b_ssel, b_scnt, b_itemy, b_pmax, b_cont_cnt, b_cont_idx, b_cont_state, b_cblk_bank,
+        b_cblk_first;
 
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
@@ -225,7 +229,8 @@ class Engine {
                 &b_chain_out, &b_queue, &b_trace, &b_red_off, &b_red_cnt, &b_red_out,
                 &b_rs_pair, &b_rs_in, &b_rs_out, &b_pairs, &b_aoff, &b_acnt, &b_ascr, &b_soff,
                 &b_fofs, &b_fcnt, &b_moff, &b_feat, &b_fgroups, &b_pred, &b_isel, &b_srow,
-                &b_sitems, &b_ssel, &b_scnt, &b_itemy, &b_pmax};
+                &b_sitems, &b_ssel, &b_scnt, &b_itemy, &b_pmax, &b_cont_cnt, &b_cont_idx,
+                &b_cont_state, &b_cblk_bank, &b_cblk_first};
     }
 
     template <class T>
@@ -456,7 +461,8 @@ void Engine::screen_seeds(std::vector<BankDesc> &banks, const SeedDesc *seeds_de
     const int *dbf = upload(b_sblk_first, blk_first);
     Hit *dc = b_scands.get<Hit>(cand);
     launch_bank(s_, A_, db, dbb, dbf, (int)blk_bank.size(), seeds_dev, b_masks.get<uint8_t>(1),
-                b_list_pq.get<P8>(1), b_list_na.get<int>(1), list_cap_, dc, fast_);
+                b_list_pq.get<P8>(1), b_list_na.get<int>(1), list_cap_, dc, fast_, 0, nullptr,
+                nullptr, nullptr);
     launched();
     float *sc = b_pred.get<float>(cand);
     launch_hit_scores(s_, dc, cand, 1, sc);
@@ -521,11 +527,41 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
                            b_lists_n.get<int>(1), b_list_na.get<int>(1), list_cap_, lpq);
         launched();
     }
+    // Candidates still refining after bank_pass_cap passes run on in a second,
+    // compacted launch (their bank's lanes would otherwise idle up to B.iters).
+    bool cont = false;
+    if (bank_pass_cap > 0)
+        for (const BankDesc &b : banks) cont = cont || b.iters > bank_pass_cap;
+    int *ccnt = nullptr, *cidx = nullptr;
+    long long *cst = nullptr;
+    if (cont) {
+        ccnt = b_cont_cnt.get<int>(banks.size());
+        cidx = b_cont_idx.get<int>(cand);
+        cst = b_cont_state.get<long long>(cand);  // one (last count, next pass) pair per slot
+        IA_CHECK(hipMemsetAsync(ccnt, 0, banks.size() * sizeof(int), s_));
+    }
     IA_CHECK(hipEventRecord(e0_, s_));
     launch_bank(s_, A_, db, dbb, dbf, (int)blk_bank.size(), ds, b_masks.get<uint8_t>(1), lpq,
-                b_list_na.get<int>(1), list_cap_, dc, fast_);
+                b_list_na.get<int>(1), list_cap_, dc, fast_, cont ? bank_pass_cap : 0, ccnt, cidx,
+                cst);
     launched();
     IA_CHECK(hipGetLastError());
+    if (cont) {
+        const std::vector<int> cnt = download(ccnt, banks.size());
+        std::vector<int> cb, cf;
+        for (size_t b = 0; b < banks.size(); b++)
+            for (int k = 0; k < cnt[b]; k += 256) {
+                cb.push_back((int)b);
+                cf.push_back(k);
+            }
+        if (!cb.empty()) {
+            launch_bank_cont(s_, A_, db, upload(b_cblk_bank, cb), upload(b_cblk_first, cf),
+                             (int)cb.size(), lpq, b_list_na.get<int>(1), list_cap_, dc, fast_,
+                             ccnt, cidx, cst);
+            launched();
+            IA_CHECK(hipGetLastError());
+        }
+    }
     IA_CHECK(hipEventRecord(e1_, s_));
     launch_bank_reduce(s_, db, (int)banks.size(), b_list_na.get<int>(1), dc, out_dev);
     launched();
@@ -1468,19 +1504,20 @@ int g_top_seeds = 0;
 unsigned g_ranker_ver = 0;
 unsigned g_engine_ranker_ver = 0;
 
-// Chain-kernel launch shape: {waves per SIMD, slots per CU, trace placement}
-// (Engine::chain_*).  Read once from a tuning file written by the autotuner,
+// Launch shape: {chain waves per SIMD, chain slots per CU, chain trace
+// placement, seed-bank pass cap} (Engine::chain_*, Engine::bank_pass_cap).  Read once from a tuning file written by the autotuner,
 // then settable with indialign_set_launch_tuning.
-int g_tune[3] = {3, 0, 0};
+int g_tune[4] = {3, 0, 0, 5};
 bool g_tune_loaded = false;
 
-bool tune_valid(int waves, int slots, int trace) {
-    return (waves == 2 || waves == 3) && slots >= 0 && slots <= 64 && trace >= 0 && trace <= 2;
+bool tune_valid(int waves, int slots, int trace, int cap) {
+    return (waves == 2 || waves == 3) && slots >= 0 && slots <= 64 && trace >= 0 && trace <= 2 &&
+           cap >= 0 && cap <= 64;
 }
 
 // The tuning file is $INDIALIGN_TUNING, or indialign_tuning.txt beside this
-// library; lines "chain_waves 3", "chain_slots_per_cu 0", "chain_trace 0"
-// ('#' starts a comment).  A missing file keeps the defaults; a malformed
+// library; lines "chain_waves 3", "chain_slots_per_cu 0", "chain_trace 0",
+// "bank_pass_cap 0" ('#' starts a comment).  A missing file keeps the defaults; a malformed
 // one is reported and ignored.
 void load_tuning_file() {
     if (g_tune_loaded) return;
@@ -1499,7 +1536,7 @@ void load_tuning_file() {
     }
     FILE *f = std::fopen(path.c_str(), "r");
     if (!f) return;
-    int t[3] = {g_tune[0], g_tune[1], g_tune[2]};
+    int t[4] = {g_tune[0], g_tune[1], g_tune[2], g_tune[3]};
     char line[256];
     bool ok = true;
     while (std::fgets(line, sizeof line, f)) {
@@ -1510,9 +1547,10 @@ void load_tuning_file() {
         if (!std::strcmp(key, "chain_waves")) t[0] = v;
         else if (!std::strcmp(key, "chain_slots_per_cu")) t[1] = v;
         else if (!std::strcmp(key, "chain_trace")) t[2] = v;
+        else if (!std::strcmp(key, "bank_pass_cap")) t[3] = v;
     }
     std::fclose(f);
-    if (ok && tune_valid(t[0], t[1], t[2])) {
+    if (ok && tune_valid(t[0], t[1], t[2], t[3])) {
         std::memcpy(g_tune, t, sizeof t);
     } else {
         std::fprintf(stderr, "indialign_amd: ignoring malformed tuning file %s\n", path.c_str());
@@ -1525,6 +1563,7 @@ void sync_tuning() {
     g_engine->chain_waves = g_tune[0];
     g_engine->chain_slots_per_cu = g_tune[1];
     g_engine->chain_trace = g_tune[2];
+    g_engine->bank_pass_cap = g_tune[3];
 }
 
 // With g_mu held and g_engine alive: install the current ranker.
@@ -1716,20 +1755,22 @@ int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets,
     return 0;
 }
 
-int indialign_set_launch_tuning(int chain_waves, int chain_slots_per_cu, int chain_trace) {
-    if (!tune_valid(chain_waves, chain_slots_per_cu, chain_trace)) return -1;
+int indialign_set_launch_tuning(int chain_waves, int chain_slots_per_cu, int chain_trace,
+                                int bank_pass_cap) {
+    if (!tune_valid(chain_waves, chain_slots_per_cu, chain_trace, bank_pass_cap)) return -1;
     std::lock_guard<std::mutex> lk(g_mu);
     g_tune_loaded = true;  // an explicit setting wins over the file
     g_tune[0] = chain_waves;
     g_tune[1] = chain_slots_per_cu;
     g_tune[2] = chain_trace;
+    g_tune[3] = bank_pass_cap;
     return 0;
 }
 
-int indialign_get_launch_tuning(int *out3) {
+int indialign_get_launch_tuning(int *out4) {
     std::lock_guard<std::mutex> lk(g_mu);
     load_tuning_file();
-    for (int k = 0; k < 3; k++) out3[k] = g_tune[k];
+    for (int k = 0; k < 4; k++) out4[k] = g_tune[k];
     return 0;
 }
 

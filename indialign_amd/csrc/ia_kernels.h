@@ -86,10 +86,21 @@ struct ListOut {
 };
 
 // Launchers (defined in the .hip files; all asynchronous on `s`).
+// pass_cap > 0: candidates still refining after pass_cap passes are left for
+// launch_bank_cont (their index appended to cont_idx[cand_off + k] with
+// k < cont_cnt[bank], which must start at zero; cont_state holds 8 bytes per
+// candidate slot).
 void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
                  const int *blk_first, int n_blocks, const SeedDesc *seeds,
                  const uint8_t *masks, const P8 *list_pq, const int *list_na, int list_cap,
-                 Hit *cands, bool fast);
+                 Hit *cands, bool fast, int pass_cap, int *cont_cnt, int *cont_idx,
+                 void *cont_state);
+// The rest of those candidates: block b runs entries blk_first[b] .. +255 of
+// bank blk_bank[b]'s list.
+void launch_bank_cont(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
+                      const int *blk_first, int n_blocks, const P8 *list_pq, const int *list_na,
+                      int list_cap, Hit *cands, bool fast, const int *cont_cnt,
+                      const int *cont_idx, const void *cont_state);
 // Interleaved copies of the traceback lists of the PTS_LIST banks, read by
 // launch_bank (list_pq[slot * list_cap + k]).
 void launch_list_gather(hipStream_t s, const Arena &A, const BankDesc *banks, int n_banks,

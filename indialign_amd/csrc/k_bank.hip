@@ -53,12 +53,97 @@ IA_D int bank_live_seeds(const BankDesc &B, int n) {
     return B.sel_m > 0 && B.sel_m < k ? B.sel_m : k;
 }
 
+// Refinement state of a candidate stopped at the bank's pass cap: the count
+// of its last pass and the index of its next pass (its R, t go to its
+// candidate slot).
+struct ContState {
+    int last, it;
+};
+
+// Passes it .. it_end-1 of the d0s-cutoff refinement (search.h:38-55): select
+// the points within the cutoff, re-superpose on >= 3, stop when the count
+// repeats.  Returns true when the candidate's refinement has finished (the
+// count repeated or all B.iters passes ran), false when it stopped at it_end.
+template <int PTS, bool FAST>
+IA_D bool refine_passes(const Arena &A, const BankDesc &B, int n, int c, const P8 *lq, double *R,
+                        double *t, int &it, int &last, int it_end) {
+    const int off = A.info[B.pair].off;
+    const double cut = B.cut[c];
+    const double lim = cut * cut;
+    KabschSeq ks;
+    for (; it < it_end; it++) {
+        ks.reset();
+        int cnt = 0;
+        if (FAST) {  // one pass: selection + raw moments
+            for (int i = 0; i < n; i++) {
+                double x, y, z, qa, qb, qc;
+                if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+                if (dist2_moved_fma(R, t, x, y, z, qa, qb, qc) <= lim) {
+                    cnt++;
+                    ks.add_moment(x, y, z, qa, qb, qc);
+                }
+            }
+            if (cnt >= 3) {
+                ks.finish_moments();
+                ks.template solve<FAST>(R, t);
+            }
+        } else {
+            for (int i = 0; i < n; i++) {
+                double x, y, z, qa, qb, qc;
+                if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                    cnt++;
+                    ks.add_centroid(x, y, z, qa, qb, qc);
+                }
+            }
+            if (cnt >= 3) {
+                ks.centre();
+                for (int i = 0; i < n; i++) {
+                    double x, y, z, qa, qb, qc;
+                    if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+                    if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) ks.add_cov(x, y, z, qa, qb, qc);
+                }
+                ks.template solve<FAST>(R, t);
+            }
+        }
+        if (cnt == last) {
+            it++;
+            return true;
+        }
+        last = cnt;
+    }
+    return it >= B.iters;
+}
+
+// TM-score of (R, t) over the bank's point set (core.h:134-166).
+template <int PTS, bool FAST>
+IA_D double candidate_score(const Arena &A, const BankDesc &B, int n, const P8 *lq,
+                            const double *R, const double *t) {
+    const PairInfo &pi = A.info[B.pair];
+    const int off = pi.off;
+    const double d0sq = fmax(pi.d0 * pi.d0, 1e-12), c8 = B.sd8 * B.sd8;
+    double acc = 0.0;
+    for (int i = 0; i < n; i++) {
+        double x, y, z, qa, qb, qc;
+        if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+        const double d2 = FAST ? dist2_moved_fma(R, t, x, y, z, qa, qb, qc)
+                               : dist2_moved(R, t, x, y, z, qa, qb, qc);
+        if (B.use_d8 && !(d2 <= c8)) continue;
+        // fast: one divide, d0^2 / (d0^2 + d2), instead of the reference's two
+        acc += FAST ? d0sq / (d0sq + d2) : tm_term(d2, d0sq);
+    }
+    return acc / fmax(pi.Lnorm, 1.0);
+}
+
 // One candidate: iterative_seed_refine (search.h:28-56) or the gapless
 // offset start + refine_from_rt (threading.h:15-37, 415-430), then the
 // TM-score of search.h:358-362.
+// Returns true with out = the finished candidate, or false when its
+// refinement stopped at pass_cap (> 0): out holds its (R, t), st the rest.
 template <int PTS, bool FAST>
-IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const SeedDesc *seeds,
-                        const uint8_t *masks, const P8 *lq, Hit &out) {
+IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const SeedDesc *seeds,
+                        const uint8_t *masks, const P8 *lq, int pass_cap, Hit &out,
+                        ContState &st) {
     const PairInfo &pi = A.info[B.pair];
     const int off = pi.off;
     const int C = B.C;
@@ -92,7 +177,7 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         if (ks.cnt < (double)B.min_ali) {
             out.score = -INFINITY;
             set_identity(out.R, out.t);
-            return;
+            return true;
         }
         if (FAST) {
             ks.finish_moments();
@@ -136,91 +221,113 @@ IA_D void run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         }
     }
     // d0s-cutoff refinement; stop when the inlier count repeats.
-    const double cut = B.cut[c];
-    const double lim = cut * cut;
-    int last = -1;
-    for (int it = 0; it < B.iters; it++) {
-        ks.reset();
-        int cnt = 0;
-        if (FAST) {  // one pass: selection + raw moments
-            for (int i = 0; i < n; i++) {
-                double x, y, z, qa, qb, qc;
-                if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
-                if (dist2_moved_fma(R, t, x, y, z, qa, qb, qc) <= lim) {
-                    cnt++;
-                    ks.add_moment(x, y, z, qa, qb, qc);
-                }
-            }
-            if (cnt >= 3) {
-                ks.finish_moments();
-                ks.template solve<FAST>(R, t);
-            }
-            if (cnt == last) break;
-            last = cnt;
-            continue;
-        }
-        for (int i = 0; i < n; i++) {
-            double x, y, z, qa, qb, qc;
-            if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
-            if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
-                cnt++;
-                ks.add_centroid(x, y, z, qa, qb, qc);
-            }
-        }
-        if (cnt >= 3) {
-            ks.centre();
-            for (int i = 0; i < n; i++) {
-                double x, y, z, qa, qb, qc;
-                if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
-                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) ks.add_cov(x, y, z, qa, qb, qc);
-            }
-            ks.template solve<FAST>(R, t);
-        }
-        if (cnt == last) break;
-        last = cnt;
-    }
-    // TM-score over the bank's point set (core.h:134-166)
-    const double d0sq = fmax(pi.d0 * pi.d0, 1e-12), c8 = B.sd8 * B.sd8;
-    double acc = 0.0;
-    for (int i = 0; i < n; i++) {
-        double x, y, z, qa, qb, qc;
-        if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
-        const double d2 = FAST ? dist2_moved_fma(R, t, x, y, z, qa, qb, qc)
-                               : dist2_moved(R, t, x, y, z, qa, qb, qc);
-        if (B.use_d8 && !(d2 <= c8)) continue;
-        // fast: one divide, d0^2 / (d0^2 + d2), instead of the reference's two
-        acc += FAST ? d0sq / (d0sq + d2) : tm_term(d2, d0sq);
-    }
+    int it = 0, last = -1;
+    const bool done = refine_passes<PTS, FAST>(A, B, n, c, lq, R, t, it, last,
+                                               pass_cap > 0 ? min(pass_cap, B.iters) : B.iters);
 #pragma unroll
     for (int q = 0; q < 9; q++) out.R[q] = R[q];
     out.t[0] = t[0]; out.t[1] = t[1]; out.t[2] = t[2];
-    out.score = acc / fmax(pi.Lnorm, 1.0);
+    if (!done) {  // stopped at the pass cap: the continuation kernel resumes it
+        st.last = last;
+        st.it = it;
+        out.score = kNeg;
+        return false;
+    }
+    out.score = candidate_score<PTS, FAST>(A, B, n, lq, R, t);
+    return true;
 }
 
+// With pass_cap > 0, a candidate still refining after pass_cap passes
+// leaves its (R, t) in its slot and its state in cont_state, and its index
+// joins its bank's list (cont_idx[cand_off + k], k < cont_cnt[bank]) for
+// bank_cont_kernel; the lanes of a wavefront then wait for at most pass_cap
+// passes of their slowest candidate instead of up to B.iters.
 template <bool FAST>
 __global__ __launch_bounds__(256) void bank_kernel(Arena A, const BankDesc *banks,
                                                    const int *blk_bank, const int *blk_first,
                                                    const SeedDesc *seeds, const uint8_t *masks,
                                                    const P8 *list_pq, const int *list_na,
-                                                   int list_cap, Hit *cands) {
-    const BankDesc &B = banks[blk_bank[blockIdx.x]];
+                                                   int list_cap, Hit *cands, int pass_cap,
+                                                   int *cont_cnt, int *cont_idx,
+                                                   ContState *cont_state) {
+    const int bank = blk_bank[blockIdx.x];
+    const BankDesc &B = banks[bank];
     const int j = blk_first[blockIdx.x] + (int)threadIdx.x;
     if (j >= B.cand_cap) return;
     Hit h;
+    ContState st{-1, 0};
+    bool done = true;
     const int n = bank_points(B, list_na);
     const int K = bank_live_seeds(B, n);
     if (j >= K * B.C || n <= 0) {
         h.score = -INFINITY;
         set_identity(h.R, h.t);
     } else if (B.pts == PTS_FULL) {
-        run_candidate<PTS_FULL, FAST>(A, B, n, j, seeds, masks, nullptr, h);
+        done = run_candidate<PTS_FULL, FAST>(A, B, n, j, seeds, masks, nullptr, pass_cap, h, st);
     } else if (B.pts == PTS_LIST) {
         const long long base = (long long)B.a0 * list_cap;
-        run_candidate<PTS_LIST, FAST>(A, B, n, j, seeds, masks, list_pq + base, h);
+        done = run_candidate<PTS_LIST, FAST>(A, B, n, j, seeds, masks, list_pq + base, pass_cap, h,
+                                             st);
     } else {
-        run_candidate<PTS_CWIN, FAST>(A, B, n, j, seeds, masks, nullptr, h);
+        done = run_candidate<PTS_CWIN, FAST>(A, B, n, j, seeds, masks, nullptr, pass_cap, h, st);
     }
     cands[B.cand_off + j] = h;
+    if (pass_cap <= 0) return;
+    // append the unfinished candidates of this wavefront (one atomic per wave)
+    const unsigned long long m = __ballot(!done);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int lead = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == lead) base = atomicAdd(&cont_cnt[bank], __popcll(m));
+    base = __shfl(base, lead, 64);
+    if (!done) {
+        const int k = base + __popcll(m & ((1ull << lane) - 1ull));
+        cont_idx[B.cand_off + k] = j;
+        cont_state[B.cand_off + j] = st;
+    }
+}
+
+// The unfinished candidates of each bank, 256 per block: the rest of their
+// refinement (from their saved state) and their TM-score.
+template <bool FAST>
+__global__ __launch_bounds__(256) void bank_cont_kernel(Arena A, const BankDesc *banks,
+                                                        const int *blk_bank, const int *blk_first,
+                                                        const P8 *list_pq, const int *list_na,
+                                                        int list_cap, Hit *cands,
+                                                        const int *cont_cnt, const int *cont_idx,
+                                                        const ContState *cont_state) {
+    const int bank = blk_bank[blockIdx.x];
+    const BankDesc &B = banks[bank];
+    const int k = blk_first[blockIdx.x] + (int)threadIdx.x;
+    if (k >= cont_cnt[bank]) return;
+    const int j = cont_idx[B.cand_off + k];
+    Hit &h = cands[B.cand_off + j];
+    ContState st = cont_state[B.cand_off + j];
+    double R[9], t[3];
+#pragma unroll
+    for (int q = 0; q < 9; q++) R[q] = h.R[q];
+    t[0] = h.t[0]; t[1] = h.t[1]; t[2] = h.t[2];
+    const int n = bank_points(B, list_na);
+    const int c = j % B.C;
+    double score;
+    if (B.pts == PTS_FULL) {
+        refine_passes<PTS_FULL, FAST>(A, B, n, c, nullptr, R, t, st.it, st.last, B.iters);
+        score = candidate_score<PTS_FULL, FAST>(A, B, n, nullptr, R, t);
+    } else if (B.pts == PTS_LIST) {
+        const P8 *lq = list_pq + (long long)B.a0 * list_cap;
+        refine_passes<PTS_LIST, FAST>(A, B, n, c, lq, R, t, st.it, st.last, B.iters);
+        score = candidate_score<PTS_LIST, FAST>(A, B, n, lq, R, t);
+    } else {
+        refine_passes<PTS_CWIN, FAST>(A, B, n, c, nullptr, R, t, st.it, st.last, B.iters);
+        score = candidate_score<PTS_CWIN, FAST>(A, B, n, nullptr, R, t);
+    }
+    Hit o;
+#pragma unroll
+    for (int q = 0; q < 9; q++) o.R[q] = R[q];
+    o.t[0] = t[0]; o.t[1] = t[1]; o.t[2] = t[2];
+    o.score = score;
+    h = o;
 }
 
 // Copy the aligned pairs of every PTS_LIST bank's traceback list into
@@ -447,14 +554,34 @@ __global__ __launch_bounds__(64) void weighted_kernel(Arena A, const int *pairs,
 void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
                  const int *blk_first, int n_blocks, const SeedDesc *seeds,
                  const uint8_t *masks, const P8 *list_pq, const int *list_na, int list_cap,
-                 Hit *cands, bool fast) {
+                 Hit *cands, bool fast, int pass_cap, int *cont_cnt, int *cont_idx,
+                 void *cont_state) {
     if (n_blocks <= 0) return;
+    ContState *cs = (ContState *)cont_state;
     if (fast)
         bank_kernel<true><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks,
-                                                   list_pq, list_na, list_cap, cands);
+                                                   list_pq, list_na, list_cap, cands, pass_cap,
+                                                   cont_cnt, cont_idx, cs);
     else
         bank_kernel<false><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks,
-                                                    list_pq, list_na, list_cap, cands);
+                                                    list_pq, list_na, list_cap, cands, pass_cap,
+                                                    cont_cnt, cont_idx, cs);
+}
+
+void launch_bank_cont(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
+                      const int *blk_first, int n_blocks, const P8 *list_pq, const int *list_na,
+                      int list_cap, Hit *cands, bool fast, const int *cont_cnt,
+                      const int *cont_idx, const void *cont_state) {
+    if (n_blocks <= 0) return;
+    const ContState *cs = (const ContState *)cont_state;
+    if (fast)
+        bank_cont_kernel<true><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, list_pq,
+                                                        list_na, list_cap, cands, cont_cnt,
+                                                        cont_idx, cs);
+    else
+        bank_cont_kernel<false><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, list_pq,
+                                                         list_na, list_cap, cands, cont_cnt,
+                                                         cont_idx, cs);
 }
 
 void launch_list_gather(hipStream_t s, const Arena &A, const BankDesc *banks, int n_banks,

@@ -1,4 +1,4 @@
-"""Launch tuning of the chain kernel (indialign_set_launch_tuning and the
+"""Launch tuning of the chain and seed-bank kernels (indialign_set_launch_tuning and the
 tuning file the autotuner writes): range checks and file parsing run on the
 CPU (the library reads the file before touching a device); on the GPU every
 shape must give the same results, bit for bit."""
@@ -32,30 +32,35 @@ def read_tuning_in_subprocess(tuning_path):
 
 def test_setter_range_checks():
     before = ia.get_launch_tuning()
-    for bad in [(1, 0, 0), (4, 0, 0), (3, -1, 0), (3, 65, 0), (3, 0, 3), (2, 0, -1)]:
+    for bad in [(1, 0, 0, 0), (4, 0, 0, 0), (3, -1, 0, 0), (3, 65, 0, 0), (3, 0, 3, 0),
+                (2, 0, -1, 0), (3, 0, 0, -1), (3, 0, 0, 65)]:
         with pytest.raises(ia.IndialignError):
             ia.set_launch_tuning(*bad)
         assert ia.get_launch_tuning() == before
-    ia.set_launch_tuning(2, 8, 2)
-    assert ia.get_launch_tuning() == (2, 8, 2)
+    ia.set_launch_tuning(2, 8, 2, 5)
+    assert ia.get_launch_tuning() == (2, 8, 2, 5)
     ia.set_launch_tuning(*before)
 
 
 def test_tuning_file_read_by_library(tmp_path):
     p = tmp_path / "t.txt"
-    autotune.write_tuning(str(p), (2, 4, 1), "test")
+    autotune.write_tuning(str(p), (2, 4, 1, 3), "test")
     got, _ = read_tuning_in_subprocess(str(p))
-    assert got == [2, 4, 1]
+    assert got == [2, 4, 1, 3]
+    # a file may set some keys only
+    p.write_text("# partial\nbank_pass_cap 4\n")
+    got, _ = read_tuning_in_subprocess(str(p))
+    assert got == [3, 0, 0, 4]  # (default cap 5 replaced)
     # malformed or out-of-range files are reported and ignored
     p.write_text("chain_waves 7\n")
     got, err = read_tuning_in_subprocess(str(p))
-    assert got == [3, 0, 0] and "malformed" in err
+    assert got == [3, 0, 0, 5] and "malformed" in err
     p.write_text("chain_waves\n")
     got, err = read_tuning_in_subprocess(str(p))
-    assert got == [3, 0, 0] and "malformed" in err
+    assert got == [3, 0, 0, 5] and "malformed" in err
     # a missing file keeps the defaults
     got, _ = read_tuning_in_subprocess(str(tmp_path / "absent.txt"))
-    assert got == [3, 0, 0]
+    assert got == [3, 0, 0, 5]
 
 
 def test_candidate_shapes():
@@ -63,12 +68,15 @@ def test_candidate_shapes():
     assert c[0] == (3, 0, 0) and len(set(c)) == len(c)
     for w, s, t in c:
         assert w in (2, 3) and 0 <= s <= 64 and t in (0, 1, 2)
+    assert autotune.BANK_CAPS[0] == 0 and all(0 <= x <= 64 for x in autotune.BANK_CAPS)
 
 
 @pytest.mark.gpu
 def test_launch_shapes_give_identical_results():
     """Short pairs plus one long masked pair, plain and fast methods: every
-    launch shape returns the same scores, rotations and translations."""
+    launch shape (chain kernel variant, slots, trace placement, seed-bank pass
+    cap with its compacted continuation launch) returns the same scores,
+    rotations and translations."""
     if not ia.gpu_available():
         pytest.fail("no GPU visible to the HIP runtime")
     before = ia.get_launch_tuning()
@@ -76,7 +84,8 @@ def test_launch_shapes_give_identical_results():
     try:
         for method in (0, 1):
             ref = None
-            for shape in [(3, 0, 0), (2, 0, 0), (3, 4, 2), (2, 8, 1), (3, 0, 1)]:
+            for shape in [(3, 0, 0, 0), (2, 0, 0, 0), (3, 4, 2, 0), (2, 8, 1, 2), (3, 0, 1, 3),
+                          (3, 0, 0, 1), (3, 0, 0, 6)]:
                 ia.set_launch_tuning(*shape)
                 out = ia.tmscore_search_batch(pairs, method=method)
                 print(f"method {method} shape {shape}: mean score {out[0].mean():.12f}", flush=True)
