@@ -77,24 +77,26 @@ IA_D float vmax(float a, float b) {
 //
 // Schedule: lane l computes rows l, l+64, l+128, ... one after another, a
 // row taking P = max(nlen, 65) steps, so row a = 64m + l holds cell (a, b)
-// at step s(a, b) = mP + l + b (fill_step).  Every cell's up and diagonal
-// operands were computed one step earlier by lane l-1 and arrive by a lane
-// shift; lane 0 reads the previous round's last row (lane 63's) from an LDS
-// buffer, which round 0 finds zeroed (the H = 0 boundary with no gap).  The
-// left operand is the lane's own previous value.  A round's first 64 steps
-// (phase A) are those where lane q = step - mP moves on to its next row;
-// the rest (phase B) need no per-lane row switch.
+// at step s(a, b) = mP + l + b (fill_row_start(a, P) + b).  Every cell's up
+// and diagonal operands were computed one step earlier by lane l-1 and
+// arrive by a lane shift; lane 0 reads the previous round's last row (lane
+// 63's) from an LDS buffer, which round 0 finds zeroed (the H = 0 boundary
+// with no gap).  The left operand is the lane's own previous value.  A
+// round's first 64 steps (phase A) are those where lane q = step - mP moves
+// on to its next row; the rest (phase B, two steps per loop iteration) need
+// no per-lane row switch.
 //
 // Trace codes (0 diag, 1 up, 2 left) are packed by step: lane l shifts each
 // code into its word, and every 16 steps the wavefront stores its 64 words
 // side by side (word (s >> 4) * 64 + l), one coalesced 256-byte store; the
-// traceback finds cell (a, b) from its step s(a, b) alone.  H = max(diag, up, left) is exact; ties only
-// decide the trace code; G = H + gap_open for diagonally entered cells, else
-// H, is what down and right neighbours add.
+// traceback finds cell (a, b) from its step s(a, b) alone.
+// H = max(diag, up, left) is exact; ties only decide the trace code;
+// G = H + gap_open for diagonally entered cells, else H, is what down and
+// right neighbours add.
 IA_HD int fill_period(int nlen) { return nlen > 65 ? nlen : 65; }
 IA_HD int fill_row_start(int a, int P) { return (a >> 6) * P + (a & 63); }
-// steps of a plen x nlen fill, and its trace words (16 steps x 64 lanes per
-// 64 words; word (s >> 4) * 64 + l holds lane l's codes of steps s & ~15 ..)
+// Steps of a plen x nlen fill, and its trace words: word (s >> 4) * 64 + l
+// holds lane l's codes of steps (s & ~15) .. (s | 15).
 IA_HD int fill_steps(int plen, int nlen) {
     const int M = (plen + 63) >> 6;
     return (M - 1) * fill_period(nlen) + (plen - 1 - 64 * (M - 1)) + nlen;
@@ -143,8 +145,9 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     next_row(lane);
     V mx = 0, my = 0, mz = 0;
     char sp = 0;
-    int a = lane - 64, b = P - lane;        // current row and column (lane l
-                                            // reaches b = P, its first row, at step l)
+    // current row and column; lane l's column counter reaches P at step l,
+    // where phase A starts its first row (a = l, b = 0)
+    int a = lane - 64, b = P - lane;
     V H = 0, G = 0, D = 0;
     unsigned tw = 0;
     // column operands of the current step.  FP64: loaded per lane one step
