@@ -1,0 +1,93 @@
+"""The torch-tensor entry for the scoring drivers (indialign_amd.torch_ops):
+argument checks and the d0 rule run on the CPU (they precede any device
+call); on the GPU the adapter returns the C search's scores for the same
+padded inputs."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import indialign_amd as ia
+import libs
+from indialign_amd import synth, torch_ops
+
+
+def oracle_d0(L):
+    o = np.zeros(3)
+    lib = libs.oracle()
+    lib.oracle_d0(ctypes.c_double(L), o.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return o
+
+
+def test_d0_rule_matches_the_search():
+    """The adapter's check uses the search's d0 / d0_search / score_d8 rule
+    (pinned against the oracle's, which the closed-form tests pin)."""
+    for L in [1, 3, 11, 11.5, 12, 15, 16, 19, 20, 23, 24, 29, 29.9, 30, 47, 64, 150, 300, 1500]:
+        np.testing.assert_allclose(torch_ops.d0_params(L), oracle_d0(L), rtol=1e-14, atol=0)
+
+
+def padded_batch(pairs, N):
+    B = len(pairs)
+    pred = torch.zeros(B, N, 3, dtype=torch.float64)
+    native = torch.zeros(B, N, 3, dtype=torch.float64)
+    pv = torch.zeros(B, N, dtype=torch.bool)
+    nv = torch.zeros(B, N, dtype=torch.bool)
+    for b, p in enumerate(pairs):
+        n = len(p.pred)
+        pred[b, :n] = torch.from_numpy(p.pred)
+        native[b, :n] = torch.from_numpy(p.native)
+        pv[b, :n] = torch.from_numpy(p.pvalid.astype(bool))
+        nv[b, :n] = torch.from_numpy(p.nvalid.astype(bool))
+    L = torch.tensor([p.lnorm for p in pairs], dtype=torch.float64)
+    d = torch.tensor([torch_ops.d0_params(float(x)) for x in L], dtype=torch.float32)
+    return pred, native, pv & nv, d[:, 0], d[:, 1], d[:, 2], L.float(), pv, nv
+
+
+def test_argument_checks_without_device():
+    pairs = [synth.make_pair(5, n=40, sigma=1.0)]
+    pred, native, valid, d0, d0s, d8, L, pv, nv = padded_batch(pairs, 48)
+    with pytest.raises(ValueError):
+        torch_ops.tmscore_search(pred[0], native[0], valid, d0, d0s, d8, L)
+    with pytest.raises(ValueError):
+        torch_ops.tmscore_search(pred, native[:, :40], valid, d0, d0s, d8, L)
+    with pytest.raises(ValueError):
+        torch_ops.tmscore_search(pred, native, valid[:, :40], d0, d0s, d8, L)
+    with pytest.raises(ValueError):
+        torch_ops.tmscore_search(pred, native, valid, d0, d0s, d8, L, pred_valid=pv[:, :3])
+    with pytest.raises(ValueError):
+        torch_ops.tmscore_search(pred, native, valid, d0[:0], d0s, d8, L)
+    # a d0 rule other than the search's is refused
+    with pytest.raises(ValueError, match="RNA rule"):
+        torch_ops.tmscore_search(pred, native, valid, d0 * 1.1, d0s, d8, L)
+    # empty batch
+    out = torch_ops.tmscore_search(pred[:0], native[:0], valid[:0], d0[:0], d0s[:0], d8[:0], L[:0])
+    assert out.shape == (0,) and out.dtype == pred.dtype
+
+
+@pytest.mark.gpu
+def test_adapter_matches_batch_entry_and_oracle():
+    """Padded batch (N 40-120 in a 128-row tensor, masks with dropped
+    residues, float32 d0 tensors as the reference computes them): the
+    adapter's scores equal the batch entry's on the same padded arrays and
+    agree with the oracle within the parity bar."""
+    if not ia.gpu_available():
+        pytest.fail("no GPU visible to the HIP runtime")
+    pairs = [synth.make_pair(40000 + k, n_range=(40, 120), drop_frac=0.1 if k % 2 else 0.0)
+             for k in range(6)]
+    args = padded_batch(pairs, 128)
+    pred, native, valid, d0, d0s, d8, L, pv, nv = args
+    for dp_iter in (0, 1):
+        got = torch_ops.tmscore_search(pred, native, valid, d0, d0s, d8, L, dp_iter=dp_iter,
+                                       pred_valid=pv, native_valid=nv)
+        assert got.dtype == pred.dtype and got.shape == (6,)
+        tup = [(pred[b].numpy(), native[b].numpy(), valid[b].numpy().astype(np.uint8),
+                pv[b].numpy().astype(np.uint8), nv[b].numpy().astype(np.uint8), float(L[b]))
+               for b in range(6)]
+        cfg = ia.default_config(dp_iter=dp_iter)
+        want, _, _ = ia.tmscore_search_batch(tup, cfg)
+        assert np.array_equal(got.numpy(), want)
+        opairs = [synth.Pair(*t, 0, 0.0) for t in tup]
+        ref = libs.oracle_search(opairs, cfg=libs.oracle_config(dp_iter=dp_iter), serial_bank=1)
+        print(f"dp_iter {dp_iter}: adapter {got.numpy()} oracle {ref[:, 12]}", flush=True)
+        assert np.abs(got.numpy() - ref[:, 12]).max() <= 1e-6
