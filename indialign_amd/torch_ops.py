@@ -110,3 +110,44 @@ def tmscore_search(pred, native, valid, d0, d0_search, score_d8, Lnorm, max_iter
                          max_mem_gb=float(max_mem_gb), dp_iter=int(dp_iter))
     scores, _, _ = tmscore_search_batch(pairs, cfg, method=int(method))
     return torch.as_tensor(scores, dtype=pred.dtype, device=pred.device)
+
+
+def score_pair_batches(pred_coords, pred_valid, native_coords, native_valid, pair_chunk: int = 256,
+                       max_iter=None, use_fragment_search: bool = True, max_mem_gb: float = 20.0,
+                       dp_iter: int = 1, method: int = METHOD_PLAIN) -> float:
+    """Best TM-score over every (pred frame, native frame) pair, with the
+    argument meaning of the reference's multi-frame scorer
+    (scoring/_local_validation_gpu/scoring.py:15-70): pred_coords (P, N, 3)
+    with pred_valid (P, N), native_coords (F, N, 3) with native_valid (F, N);
+    a frame pair is scored on the same-index mask pred_valid & native_valid
+    with Lnorm = its native frame's valid count, and pairs with Lnorm <= 2
+    score 0.  Defaults follow that driver's (dp_iter 1,
+    scoring/_local_validation_gpu/runtime.py:13-16).  All P x F pairs go to
+    the device as one batch (pair_chunk is accepted for signature
+    compatibility; device memory holds the batch).  Returns a float, 0.0 for
+    no frames."""
+    import torch
+
+    P, F = int(pred_coords.shape[0]), int(native_coords.shape[0])
+    if P == 0 or F == 0:
+        return 0.0
+    if pred_coords.shape[1:] != native_coords.shape[1:] or pred_coords.shape[2] != 3:
+        raise ValueError("pred and native frames must be shape (P, N, 3) and (F, N, 3).")
+    if pred_valid.shape != pred_coords.shape[:2] or native_valid.shape != native_coords.shape[:2]:
+        raise ValueError("valid masks must be shape (P, N) and (F, N).")
+    pidx = torch.arange(P * F, device=pred_coords.device) // F
+    nidx = torch.arange(P * F, device=pred_coords.device) % F
+    p = pred_coords.index_select(0, pidx)
+    n = native_coords.index_select(0, nidx)
+    nv = native_valid.index_select(0, nidx).bool()
+    v = pred_valid.index_select(0, pidx).bool() & nv
+    L = nv.sum(dim=1).to(torch.float64)
+    good = L > 2.0
+    if not bool(good.any()):
+        return 0.0
+    Lg = L[good]
+    d = torch.tensor([d0_params(float(x)) for x in Lg], dtype=torch.float64)
+    s = tmscore_search(p[good], n[good], v[good], d[:, 0], d[:, 1], d[:, 2], Lg, max_iter=max_iter,
+                       use_fragment_search=use_fragment_search, max_mem_gb=max_mem_gb,
+                       dp_iter=dp_iter, method=method)
+    return max(0.0, float(s.max()))

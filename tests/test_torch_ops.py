@@ -91,3 +91,48 @@ def test_adapter_matches_batch_entry_and_oracle():
         ref = libs.oracle_search(opairs, cfg=libs.oracle_config(dp_iter=dp_iter), serial_bank=1)
         print(f"dp_iter {dp_iter}: adapter {got.numpy()} oracle {ref[:, 12]}", flush=True)
         assert np.abs(got.numpy() - ref[:, 12]).max() <= 1e-6
+
+
+def test_pair_batches_trivial_cases():
+    """No frames -> 0.0; frame pairs whose native frame has <= 2 valid
+    residues score 0 without a search."""
+    z3 = torch.zeros(0, 10, 3, dtype=torch.float64)
+    z = torch.zeros(0, 10, dtype=torch.bool)
+    p = torch.randn(2, 10, 3, dtype=torch.float64)
+    v = torch.ones(2, 10, dtype=torch.bool)
+    assert torch_ops.score_pair_batches(z3, z, p, v) == 0.0
+    assert torch_ops.score_pair_batches(p, v, z3, z) == 0.0
+    nv = torch.zeros(2, 10, dtype=torch.bool)
+    nv[:, :2] = True
+    assert torch_ops.score_pair_batches(p, v, p, nv) == 0.0
+    with pytest.raises(ValueError):
+        torch_ops.score_pair_batches(p, v[:, :5], p, v)
+
+
+@pytest.mark.gpu
+def test_pair_batches_is_the_best_frame_pair():
+    """3 pred frames x 2 native frames of one target (noisy copies, one
+    native frame with dropped residues): the result is the maximum of the
+    six single-pair searches, each checked against the oracle."""
+    if not ia.gpu_available():
+        pytest.fail("no GPU visible to the HIP runtime")
+    base = synth.make_pair(41000, n=90, sigma=2.0)
+    rng = np.random.default_rng(7)
+    preds = np.stack([base.pred + rng.normal(0, s, base.pred.shape) for s in (0.5, 3.0, 8.0)])
+    natives = np.stack([base.native, base.native + rng.normal(0, 1.0, base.native.shape)])
+    pv = np.ones((3, 90), bool)
+    pv[1, 10:20] = False
+    nv = np.ones((2, 90), bool)
+    nv[1, 40:50] = False
+    got = torch_ops.score_pair_batches(torch.from_numpy(preds), torch.from_numpy(pv),
+                                       torch.from_numpy(natives), torch.from_numpy(nv))
+    singles = []
+    for i in range(3):
+        for j in range(2):
+            m = (pv[i] & nv[j]).astype(np.uint8)
+            singles.append(synth.Pair(preds[i], natives[j], m, m, m, float(nv[j].sum()), 0, 0.0))
+    ref = libs.oracle_search(singles, serial_bank=1)
+    s, _, _ = ia.tmscore_search_batch(singles)
+    print(f"frames: best {got} singles {s} oracle {ref[:, 12]}", flush=True)
+    assert got == float(s.max())
+    assert np.abs(s - ref[:, 12]).max() <= 1e-6
