@@ -64,9 +64,14 @@ struct ContState {
 // the points within the cutoff, re-superpose on >= 3, stop when the count
 // repeats.  Returns true when the candidate's refinement has finished (the
 // count repeated or all B.iters passes ran), false when it stopped at it_end.
+// Selection bits of the plain refinement's first pass, points 0..kSelCache-1:
+// word w of thread t at sel_cache[w * 256 + t] (8 KB per 256-thread block).
+constexpr int kSelCache = 256;
+
 template <int PTS, bool FAST>
 IA_D bool refine_passes(const Arena &A, const BankDesc &B, int n, int c, const P8 *lq, double *R,
                         double *t, int &it, int &last, int it_end) {
+    __shared__ unsigned sel_cache[(kSelCache / 32) * 256];
     const int off = A.info[B.pair].off;
     const double cut = B.cut[c];
     const double lim = cut * cut;
@@ -88,18 +93,38 @@ IA_D bool refine_passes(const Arena &A, const BankDesc &B, int n, int c, const P
                 ks.template solve<FAST>(R, t);
             }
         } else {
+            // two passes (centroids, then centred covariance); the first
+            // records its selection of points 0..kSelCache-1 so the second
+            // recomputes distances only beyond them
+            unsigned word = 0;
             for (int i = 0; i < n; i++) {
                 double x, y, z, qa, qb, qc;
-                if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
-                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                bool in = false;
+                if (get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc) &&
+                    dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                    in = true;
                     cnt++;
                     ks.add_centroid(x, y, z, qa, qb, qc);
+                }
+                if (i < kSelCache) {
+                    word |= (in ? 1u : 0u) << (i & 31);
+                    if ((i & 31) == 31 || i == n - 1) {
+                        sel_cache[(i >> 5) * 256 + threadIdx.x] = word;
+                        word = 0;
+                    }
                 }
             }
             if (cnt >= 3) {
                 ks.centre();
                 for (int i = 0; i < n; i++) {
                     double x, y, z, qa, qb, qc;
+                    if (i < kSelCache) {
+                        if ((i & 31) == 0) word = sel_cache[(i >> 5) * 256 + threadIdx.x];
+                        const bool in = (word >> (i & 31)) & 1u;
+                        get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+                        if (in) ks.add_cov(x, y, z, qa, qb, qc);
+                        continue;
+                    }
                     if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
                     if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) ks.add_cov(x, y, z, qa, qb, qc);
                 }
