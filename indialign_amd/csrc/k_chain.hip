@@ -455,6 +455,10 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
             }
             continue;
         }
+        // lane s's pair base + s + 8k is its pair (base >> 3) + k: with at most
+        // 64 per lane, the first pass leaves its selection in a bit mask
+        const bool keep_sel = na <= 8 * 64;
+        unsigned long long selmask = 0;
         for (int base = 0; base < na; base += 8 * kLiteBatch) {
             Batch B;
             load(base, !done, B);
@@ -462,6 +466,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
             for (int k = 0; k < kLiteBatch; k++) {
                 const bool sel = B.in[k] &&
                     dist2_moved(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]) <= lim;
+                if (keep_sel) selmask |= (unsigned long long)(sel ? 1 : 0) << ((base >> 3) + k);
                 cnt += sel ? 1 : 0;
                 c[0] += sel ? B.x[k] : 0.0; c[1] += sel ? B.y[k] : 0.0; c[2] += sel ? B.z[k] : 0.0;
                 c[3] += sel ? B.a[k] : 0.0; c[4] += sel ? B.b[k] : 0.0; c[5] += sel ? B.c[k] : 0.0;
@@ -482,8 +487,12 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
                 load(base, solve, B);
 #pragma unroll
                 for (int k = 0; k < kLiteBatch; k++) {
-                    const bool sel = B.in[k] &&
-                        dist2_moved(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]) <= lim;
+                    bool sel;
+                    if (keep_sel)
+                        sel = (selmask >> ((base >> 3) + k)) & 1ull;
+                    else
+                        sel = B.in[k] &&
+                              dist2_moved(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]) <= lim;
                     const double d0 = sel ? B.x[k] - cp[0] : 0.0, d1 = sel ? B.y[k] - cp[1] : 0.0;
                     const double d2 = sel ? B.z[k] - cp[2] : 0.0;
                     const double e0 = sel ? B.a[k] - cq[0] : 0.0, e1 = sel ? B.b[k] - cq[1] : 0.0;
