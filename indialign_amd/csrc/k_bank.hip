@@ -65,13 +65,14 @@ struct ContState {
 // repeats.  Returns true when the candidate's refinement has finished (the
 // count repeated or all B.iters passes ran), false when it stopped at it_end.
 // Selection bits of the plain refinement's first pass, points 0..kSelCache-1:
-// word w of thread t at sel_cache[w * 256 + t] (8 KB per 256-thread block).
+// word w of thread t at sel_cache[w * 256 + t].  One 8 KB array per
+// 256-thread block, shared by every point-set variant of refine_passes.
 constexpr int kSelCache = 256;
+__shared__ unsigned sel_cache[(kSelCache / 32) * 256];
 
 template <int PTS, bool FAST>
 IA_D bool refine_passes(const Arena &A, const BankDesc &B, int n, int c, const P8 *lq, double *R,
                         double *t, int &it, int &last, int it_end) {
-    __shared__ unsigned sel_cache[(kSelCache / 32) * 256];
     const int off = A.info[B.pair].off;
     const double cut = B.cut[c];
     const double lim = cut * cut;
@@ -94,10 +95,11 @@ IA_D bool refine_passes(const Arena &A, const BankDesc &B, int n, int c, const P
             }
         } else {
             // two passes (centroids, then centred covariance); the first
-            // records its selection of points 0..kSelCache-1 so the second
-            // recomputes distances only beyond them
+            // records its selection of points 0..m-1 so the second
+            // recomputes distances only for points m..n-1
+            const int m = n < kSelCache ? n : kSelCache;
             unsigned word = 0;
-            for (int i = 0; i < n; i++) {
+            for (int i = 0; i < m; i++) {
                 double x, y, z, qa, qb, qc;
                 bool in = false;
                 if (get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc) &&
@@ -106,25 +108,30 @@ IA_D bool refine_passes(const Arena &A, const BankDesc &B, int n, int c, const P
                     cnt++;
                     ks.add_centroid(x, y, z, qa, qb, qc);
                 }
-                if (i < kSelCache) {
-                    word |= (in ? 1u : 0u) << (i & 31);
-                    if ((i & 31) == 31 || i == n - 1) {
-                        sel_cache[(i >> 5) * 256 + threadIdx.x] = word;
-                        word = 0;
-                    }
+                word |= (in ? 1u : 0u) << (i & 31);
+                if ((i & 31) == 31 || i == m - 1) {
+                    sel_cache[(i >> 5) * 256 + threadIdx.x] = word;
+                    word = 0;
+                }
+            }
+            for (int i = m; i < n; i++) {
+                double x, y, z, qa, qb, qc;
+                if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
+                if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) {
+                    cnt++;
+                    ks.add_centroid(x, y, z, qa, qb, qc);
                 }
             }
             if (cnt >= 3) {
                 ks.centre();
-                for (int i = 0; i < n; i++) {
+                for (int i = 0; i < m; i++) {
                     double x, y, z, qa, qb, qc;
-                    if (i < kSelCache) {
-                        if ((i & 31) == 0) word = sel_cache[(i >> 5) * 256 + threadIdx.x];
-                        const bool in = (word >> (i & 31)) & 1u;
-                        get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc);
-                        if (in) ks.add_cov(x, y, z, qa, qb, qc);
-                        continue;
-                    }
+                    if ((i & 31) == 0) word = sel_cache[(i >> 5) * 256 + threadIdx.x];
+                    get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc);
+                    if ((word >> (i & 31)) & 1u) ks.add_cov(x, y, z, qa, qb, qc);
+                }
+                for (int i = m; i < n; i++) {
+                    double x, y, z, qa, qb, qc;
                     if (!get_pt<PTS>(A, off, lq, B.a0, B.a1, i, x, y, z, qa, qb, qc)) continue;
                     if (dist2_moved(R, t, x, y, z, qa, qb, qc) <= lim) ks.add_cov(x, y, z, qa, qb, qc);
                 }
@@ -268,7 +275,8 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
 // bank_cont_kernel; the lanes of a wavefront then wait for at most pass_cap
 // passes of their slowest candidate instead of up to B.iters.
 template <bool FAST>
-__global__ __launch_bounds__(256) void bank_kernel(Arena A, const BankDesc *banks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAST ? 4 : 5))) void bank_kernel(
+    Arena A, const BankDesc *banks,
                                                    const int *blk_bank, const int *blk_first,
                                                    const SeedDesc *seeds, const uint8_t *masks,
                                                    const P8 *list_pq, const int *list_na,
