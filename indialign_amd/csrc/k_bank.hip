@@ -65,9 +65,11 @@ struct ContState {
 // repeats.  Returns true when the candidate's refinement has finished (the
 // count repeated or all B.iters passes ran), false when it stopped at it_end.
 // Selection bits of the plain refinement's first pass, points 0..kSelCache-1:
-// word w of thread t at sel_cache[w * 256 + t].  One 8 KB array per
-// 256-thread block, shared by every point-set variant of refine_passes.
-constexpr int kSelCache = 256;
+// word w of thread t at sel_cache[w * 256 + t].  One 16 KB array per
+// 256-thread block, shared by every point-set variant of refine_passes (five
+// such blocks, the plain kernel's register-bound occupancy, use 80 KB of a
+// CU's 160 KB of LDS; a 1024-point cache costs occupancy on short pairs).
+constexpr int kSelCache = 512;
 __shared__ unsigned sel_cache[(kSelCache / 32) * 256];
 
 template <int PTS, bool FAST>
