@@ -123,6 +123,23 @@ def test_fill_round_boundaries_vs_oracle():
         assert p.pvalid.sum() != p.nvalid.sum() or p.pvalid.sum() != len(p.pvalid)
 
 
+def test_selection_cache_boundaries_vs_oracle():
+    """Point sets at the 512-point limits of the plain refinement's selection
+    caches (seed banks: bits of points 0..511 kept in LDS, later points
+    recomputed; lite search: register mask for aligned sets up to 512):
+    N = 511, 512, 513 with every residue valid, one noisy N = 512 pair whose
+    rescue chains run at that size, and a masked N = 600 pair (538 residues
+    valid on both sides)."""
+    _gpu_or_skip()
+    pairs = [synth.make_pair(40000 + n + 3, n=n, sigma=3.0) for n in (511, 512, 513)]
+    pairs.append(synth.make_pair(40522, n=512, sigma=10.0))
+    pairs.append(synth.make_pair(40600, n=600, sigma=4.0, drop_frac=0.05))  # 538 same-index
+    s, R, t = ia.tmscore_search_batch(pairs)
+    ref = libs.oracle_search(pairs, serial_bank=1)
+    _check(s, R, t, ref, "selection-cache")
+    assert [int(p.valid.sum()) for p in pairs[:3]] == [511, 512, 513]
+
+
 def test_reference_toy_pair():
     """The 5-residue toy of the reference tests (tests/test_gpu_scoring_refactor.py:179-190):
     the reference C path scores it 0.77317584653223026 for dp_iter 0 and 1."""
