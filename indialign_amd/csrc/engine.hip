@@ -574,10 +574,28 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
 }
 
 long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev, Hit *out_dev) {
+    // The queue hands out chains longest first (NW cells x iterations per
+    // chain, stable among equals), so a launch ends on its cheapest chains
+    // instead of waiting on a long one that came last.  Each chain writes
+    // only its own outputs, so the order changes no result; chain_off is
+    // the prefix in queue order.
+    std::vector<int> order(groups.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
+    auto chain_cost = [&](const ChainGroup &g) {
+        const HostPair &h = hp_[g.pair];
+        return (double)h.plen * h.nlen * g.iters;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return chain_cost(groups[a]) > chain_cost(groups[b]);
+    });
+    std::vector<ChainGroup> queued;
+    queued.reserve(groups.size());
     long long total = 0;
     int mp = 1, mn = 1, lc = 1;
-    for (ChainGroup &g : groups) {
+    for (int k : order) {
+        ChainGroup &g = groups[k];
         g.chain_off = total;
+        queued.push_back(g);
         total += g.n_chains;
         const HostPair &h = hp_[g.pair];
         mp = std::max(mp, h.plen);
@@ -591,7 +609,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         stats[2] += (double)h.plen * h.nlen * (double)filled * g.iters;
     }
     if (total == 0) return 0;
-    const ChainGroup *dg = upload(b_groups, groups);
+    const ChainGroup *dg = upload(b_groups, queued);
     int *q = b_queue.get<int>(1);
     IA_CHECK(hipMemsetAsync(q, 0, sizeof(int), s_));
     const long long trace_w = chain_trace_words(mp, mn);
