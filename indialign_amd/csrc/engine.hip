@@ -500,17 +500,30 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
         }
         screen_seeds(banks, upload(b_seeds, seeds), screen);
     }
-    std::vector<int> blk_bank, blk_first;
+    // Blocks go out in decreasing cost of their bank (points x passes; a
+    // list's length is bounded by the shorter compact side), stable among
+    // equals, so a launch ends on cheap blocks.  Every block writes only its
+    // own candidates' slots, so the order changes no result.
+    std::vector<int> bank_order(banks.size());
+    std::vector<double> bank_cost(banks.size());
     long long cand = 0;
     for (size_t b = 0; b < banks.size(); b++) {
         banks[b].cand_off = cand;
         cand += banks[b].cand_cap;
+        stats[1] += banks[b].cand_cap;
+        const HostPair &h = hp_[banks[b].pair];
+        const int n = banks[b].pts == PTS_LIST ? std::min(h.plen, h.nlen) : banks[b].n;
+        bank_cost[b] = (double)n * banks[b].iters;
+        bank_order[b] = (int)b;
+    }
+    std::stable_sort(bank_order.begin(), bank_order.end(),
+                     [&](int a, int b) { return bank_cost[a] > bank_cost[b]; });
+    std::vector<int> blk_bank, blk_first;
+    for (int b : bank_order)
         for (int j = 0; j < banks[b].cand_cap; j += 256) {
-            blk_bank.push_back((int)b);
+            blk_bank.push_back(b);
             blk_first.push_back(j);
         }
-        stats[1] += banks[b].cand_cap;
-    }
     if (!out_dev) out_dev = b_bank_out.get<Hit>(n_out);
     if (banks.empty()) return out_dev;
     const BankDesc *db = upload(b_banks, banks);
@@ -549,9 +562,9 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
     if (cont) {
         const std::vector<int> cnt = download(ccnt, banks.size());
         std::vector<int> cb, cf;
-        for (size_t b = 0; b < banks.size(); b++)
+        for (int b : bank_order)
             for (int k = 0; k < cnt[b]; k += 256) {
-                cb.push_back((int)b);
+                cb.push_back(b);
                 cf.push_back(k);
             }
         if (!cb.empty()) {
