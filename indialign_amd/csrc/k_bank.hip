@@ -66,8 +66,8 @@ struct ContState {
 // count repeated or all B.iters passes ran), false when it stopped at it_end.
 // Selection bits of the plain refinement's first pass, points 0..kSelCache-1:
 // word w of thread t at sel_cache[w * 256 + t].  One 16 KB array per
-// 256-thread block, shared by every point-set variant of refine_passes (five
-// such blocks, the plain kernel's register-bound occupancy, use 80 KB of a
+// 256-thread block, shared by every point-set variant of refine_passes (six
+// such blocks, the plain kernel's register-bound occupancy, use 96 KB of a
 // CU's 160 KB of LDS; a 1024-point cache costs occupancy on short pairs).
 constexpr int kSelCache = 512;
 __shared__ unsigned sel_cache[(kSelCache / 32) * 256];
@@ -276,15 +276,15 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
 // joins its bank's list (cont_idx[cand_off + k], k < cont_cnt[bank]) for
 // bank_cont_kernel; the lanes of a wavefront then wait for at most pass_cap
 // passes of their slowest candidate instead of up to B.iters.
+// Occupancy (measured): six wavefronts per SIMD for bank_kernel and four for
+// bank_cont_kernel, both methods; the few registers this spills cost less
+// than the latency the extra wavefronts hide.
 template <bool FAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAST ? 4 : 5))) void bank_kernel(
-    Arena A, const BankDesc *banks,
-                                                   const int *blk_bank, const int *blk_first,
-                                                   const SeedDesc *seeds, const uint8_t *masks,
-                                                   const P8 *list_pq, const int *list_na,
-                                                   int list_cap, Hit *cands, int pass_cap,
-                                                   int *cont_cnt, int *cont_idx,
-                                                   ContState *cont_state) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void bank_kernel(
+    Arena A, const BankDesc *banks, const int *blk_bank, const int *blk_first,
+    const SeedDesc *seeds, const uint8_t *masks, const P8 *list_pq, const int *list_na,
+    int list_cap, Hit *cands, int pass_cap, int *cont_cnt, int *cont_idx,
+    ContState *cont_state) {
     const int bank = blk_bank[blockIdx.x];
     const BankDesc &B = banks[bank];
     const int j = blk_first[blockIdx.x] + (int)threadIdx.x;
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAST ? 4 : 
 // The unfinished candidates of each bank, 256 per block: the rest of their
 // refinement (from their saved state) and their TM-score.
 template <bool FAST>
-__global__ __launch_bounds__(256) void bank_cont_kernel(Arena A, const BankDesc *banks,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void bank_cont_kernel(Arena A, const BankDesc *banks,
                                                         const int *blk_bank, const int *blk_first,
                                                         const P8 *list_pq, const int *list_na,
                                                         int list_cap, Hit *cands,
