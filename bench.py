@@ -168,7 +168,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(pairs are independent; see DESIGN.md for how the batch would shard)", file=sys.stderr)
+              "(pairs are independent: a multi-GPU run would split the batch by pair cost, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import indialign_amd as ia

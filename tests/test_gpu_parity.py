@@ -1,5 +1,5 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle and the
-golden fixtures written from the reference build.  Bar:
+golden fixtures written from the reference build.  Bar of the plain FP64 path:
 score within 1e-6 absolute, R and t within 1e-5, per pair."""
 import ctypes
 import os
