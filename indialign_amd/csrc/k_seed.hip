@@ -11,6 +11,11 @@ namespace {
 
 constexpr int kMaxSeen = 1024;  // existing seeds + anchors hashed per pair
 constexpr int kMaxAnchors = 5;  // indialign.cpp:74
+// Pairs of up to this many residues keep the per-residue counts, the sorted
+// candidate list (12 KB) and the current neighbourhood mask (1.5 KB) in LDS:
+// the candidate build, the sort and the mask hash are one thread's chains of
+// dependent accesses, which global memory would serve at L2 latency.
+constexpr int kLdsCand = 1536;
 
 struct CountIdx {
     int count, a;
@@ -28,17 +33,19 @@ IA_D uint64_t fnv_step(uint64_t h, uint8_t b) {
 // Pair (i, j) counts as a consistent contact at radius r: the reference
 // visits j only inside the 27 grid cells of size r around i (cell index =
 // floor(x / r) per axis), then requires pd2 <= r2, nd2 <= r2 and
-// |pd - nd| <= tol (seeds.h:206-229).
+// |pd - nd| <= tol (seeds.h:206-229).  The tests are a pure conjunction, so
+// the cheap distance test runs first and only the pairs within r pay for
+// the cell indices' divisions.
 IA_D bool contact(const Arena &A, int off, int i, int j, double r, double r2, double tol) {
     const double pix = A.px[off + i], piy = A.py[off + i], piz = A.pz[off + i];
     const double pjx = A.px[off + j], pjy = A.py[off + j], pjz = A.pz[off + j];
+    const double pdx = pix - pjx, pdy = piy - pjy, pdz = piz - pjz;
+    const double pd2 = pdx * pdx + pdy * pdy + pdz * pdz;
+    if (pd2 > r2) return false;
     const long long cx = (long long)floor(pix / r) - (long long)floor(pjx / r);
     const long long cy = (long long)floor(piy / r) - (long long)floor(pjy / r);
     const long long cz = (long long)floor(piz / r) - (long long)floor(pjz / r);
     if (cx < -1 || cx > 1 || cy < -1 || cy > 1 || cz < -1 || cz > 1) return false;
-    const double pdx = pix - pjx, pdy = piy - pjy, pdz = piz - pjz;
-    const double pd2 = pdx * pdx + pdy * pdy + pdz * pdz;
-    if (pd2 > r2) return false;
     const double ndx = A.qx[off + i] - A.qx[off + j], ndy = A.qy[off + i] - A.qy[off + j],
                  ndz = A.qz[off + i] - A.qz[off + j];
     const double nd2 = ndx * ndx + ndy * ndy + ndz * ndz;
@@ -59,6 +66,9 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
     CountIdx *cand = (CountIdx *)(counts + N);       // V (2 ints each)
     uint8_t *out = amask + aoff[blockIdx.x];
     __shared__ uint64_t seen[kMaxSeen];
+    __shared__ CountIdx lcand[kLdsCand];
+    __shared__ uint32_t lmask4[kLdsCand / 4];
+    uint8_t *lmask = (uint8_t *)lmask4;
     __shared__ double radii[6];
     __shared__ int nr, V, nseen, ncand, added, total, idx, dup;
     __shared__ uint64_t hcur;
@@ -99,6 +109,7 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
         const double r = radii[ri];
         if (r <= 0) continue;
         const double r2 = r * r, tol = fmax(0.5, r * 0.25);
+        const bool in_lds = N <= kLdsCand;  // N >= V
         for (int a = tid; a < V; a += 256) {
             const int i = vidx[a];
             int c = 1;
@@ -106,15 +117,21 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
                 const int j = vidx[b];
                 if (j != i && contact(A, off, i, j, r, r2, tol)) c++;
             }
-            counts[a] = c;
+            if (in_lds) lcand[a] = CountIdx{c, a};
+            else counts[a] = c;
         }
         __threadfence();
         __syncthreads();
+        CountIdx *cl = in_lds ? lcand : cand;
         if (tid == 0) {
+            // candidates with count >= 3 in residue order (compacted in
+            // place in LDS: entry nc <= a is written after a is read)
             int nc = 0;
-            for (int a = 0; a < V; a++)
-                if (counts[a] >= 3) cand[nc++] = CountIdx{counts[a], a};
-            ia_sort::sort(cand, cand + nc, ByCountDesc());
+            for (int a = 0; a < V; a++) {
+                const int c = in_lds ? lcand[a].count : counts[a];
+                if (c >= 3) cl[nc++] = CountIdx{c, a};
+            }
+            ia_sort::sort(cl, cl + nc, ByCountDesc());
             ncand = nc;
             added = 0;
             idx = 0;
@@ -122,15 +139,30 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
         __threadfence();
         __syncthreads();
         while (idx < ncand && added < kMaxAnchors) {
-            const int i = vidx[cand[idx].a];
+            const int i = vidx[cl[idx].a];
             uint8_t *m = out + (long long)total * N;
-            for (int j = tid; j < N; j += 256)
-                m[j] = (j == i || (A.v[off + j] && contact(A, off, i, j, r, r2, tol))) ? 1 : 0;
+            for (int j = tid; j < N; j += 256) {
+                const uint8_t x = (j == i || (A.v[off + j] && contact(A, off, i, j, r, r2, tol))) ? 1 : 0;
+                m[j] = x;
+                if (in_lds) lmask[j] = x;
+            }
             __threadfence();
             __syncthreads();
             if (tid == 0) {
                 uint64_t h = 1469598103934665603ULL;
-                for (int j = 0; j < N; j++) h = fnv_step(h, m[j]);
+                if (in_lds) {
+                    const int n4 = N >> 2;
+                    for (int w = 0; w < n4; w++) {
+                        const uint32_t x = lmask4[w];
+                        h = fnv_step(h, (uint8_t)x);
+                        h = fnv_step(h, (uint8_t)(x >> 8));
+                        h = fnv_step(h, (uint8_t)(x >> 16));
+                        h = fnv_step(h, (uint8_t)(x >> 24));
+                    }
+                    for (int j = n4 << 2; j < N; j++) h = fnv_step(h, lmask[j]);
+                } else {
+                    for (int j = 0; j < N; j++) h = fnv_step(h, m[j]);
+                }
                 hcur = h;
                 dup = 0;
             }
