@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
             if (in_lds) lcand[a] = CountIdx{c, a};
             else counts[a] = c;
         }
-        __threadfence();
+        if (!in_lds) __threadfence();  // LDS needs only the barrier
         __syncthreads();
         CountIdx *cl = in_lds ? lcand : cand;
         if (tid == 0) {
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
             added = 0;
             idx = 0;
         }
-        __threadfence();
+        if (!in_lds) __threadfence();  // LDS needs only the barrier
         __syncthreads();
         while (idx < ncand && added < kMaxAnchors) {
             const int i = vidx[cl[idx].a];
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
                 m[j] = x;
                 if (in_lds) lmask[j] = x;
             }
-            __threadfence();
+            if (!in_lds) __threadfence();  // LDS needs only the barrier
             __syncthreads();
             if (tid == 0) {
                 uint64_t h = 1469598103934665603ULL;
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
                 }
                 idx++;
             }
-            __threadfence();
+            if (!in_lds) __threadfence();  // LDS needs only the barrier
             __syncthreads();
         }
     }
