@@ -296,7 +296,7 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
     const int P = fill_period(nlen);
     typedef typename std::conditional<FAST, float, double>::type V;
     nw_fill<SMODE, TR, CL, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     __builtin_amdgcn_wave_barrier();
     // traceback (nw.h:42-68)
     if constexpr (!std::is_same<TR, unsigned>::value) {
@@ -768,7 +768,7 @@ IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, lon
             run_chain<SM_SS, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         else
             run_chain<SM_IA4, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         __builtin_amdgcn_wave_barrier();
     }
 }
