@@ -824,8 +824,10 @@ void Engine::stage_seed_bank() {
     std::vector<int> foff(n), fcnt(n);
     std::vector<long long> aoff(n), soff(n);
     long long abytes = 0, sints = 0;
+    int max_n = 0;
     for (int a = 0; a < n; a++) {
         pairs[a] = a;
+        max_n = std::max(max_n, hp_[a].N);
         foff[a] = (int)frag.size();
         fcnt[a] = (int)hp_[a].frag.size();
         if (fcnt[a] > 990) throw DevError("too many fragment seeds for one pair");
@@ -843,7 +845,7 @@ void Engine::stage_seed_bank() {
     const long long *dso = upload(b_soff, soff);
     uint8_t *masks = b_masks.get<uint8_t>(abytes);
     int *dac = b_acnt.get<int>(n);
-    launch_anchors(s_, A_, dpairs, n, dfrag, dfo, dfc, masks, dao, dac, b_ascr.get<int>(sints), dso);
+    launch_anchors(s_, A_, dpairs, n, dfrag, dfo, dfc, masks, dao, dac, b_ascr.get<int>(sints), dso, max_n);
     launched();
     IA_CHECK(hipGetLastError());
     const std::vector<int> acnt = download(dac, n);

@@ -125,7 +125,7 @@ void launch_ss(hipStream_t s, const Arena &A, int n_pairs, char *ssp, char *ssn,
 void launch_anchors(hipStream_t s, const Arena &A, const int *pairs, int n,
                     const SeedDesc *frag_seeds, const int *frag_off, const int *frag_cnt,
                     uint8_t *anchor_masks, const long long *anchor_off, int *anchor_cnt,
-                    int *scratch_i, const long long *scratch_off);
+                    int *scratch_i, const long long *scratch_off, int max_n);
 void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int n_groups,
                    long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,

@@ -69,6 +69,9 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
     __shared__ CountIdx lcand[kLdsCand];
     __shared__ uint32_t lmask4[kLdsCand / 4];
     uint8_t *lmask = (uint8_t *)lmask4;
+    // dynamic: the valid residues' pred coordinates, 3 doubles each, sized by
+    // the launch for min(longest pair, kLdsCand) residues
+    extern __shared__ double lpts[];
     __shared__ double radii[6];
     __shared__ int nr, V, nseen, ncand, added, total, idx, dup;
     __shared__ uint64_t hcur;
@@ -104,6 +107,14 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
             h = fnv_step(h, (i >= sd.a && i < sd.a + sd.b) ? A.v[off + i] : (uint8_t)0);
         seen[k] = h;
     }
+    if (N <= kLdsCand) {
+        for (int a = tid; a < V; a += 256) {
+            const int i = vidx[a];
+            lpts[3 * a] = A.px[off + i];
+            lpts[3 * a + 1] = A.py[off + i];
+            lpts[3 * a + 2] = A.pz[off + i];
+        }
+    }
     __syncthreads();
     for (int ri = 0; ri < nr; ri++) {
         const double r = radii[ri];
@@ -113,9 +124,23 @@ __global__ __launch_bounds__(256) void anchor_kernel(Arena A, const int *pairs,
         for (int a = tid; a < V; a += 256) {
             const int i = vidx[a];
             int c = 1;
-            for (int b = 0; b < V; b++) {
-                const int j = vidx[b];
-                if (j != i && contact(A, off, i, j, r, r2, tol)) c++;
+            if (in_lds) {
+                // contact()'s first test from the LDS copy (same operands
+                // and operations); the rest only for pairs within r
+                const double pix = A.px[off + i], piy = A.py[off + i], piz = A.pz[off + i];
+                for (int b = 0; b < V; b++) {
+                    const double pdx = pix - lpts[3 * b], pdy = piy - lpts[3 * b + 1],
+                                 pdz = piz - lpts[3 * b + 2];
+                    const double pd2 = pdx * pdx + pdy * pdy + pdz * pdz;
+                    if (pd2 > r2) continue;
+                    const int j = vidx[b];
+                    if (j != i && contact(A, off, i, j, r, r2, tol)) c++;
+                }
+            } else {
+                for (int b = 0; b < V; b++) {
+                    const int j = vidx[b];
+                    if (j != i && contact(A, off, i, j, r, r2, tol)) c++;
+                }
             }
             if (in_lds) lcand[a] = CountIdx{c, a};
             else counts[a] = c;
@@ -286,9 +311,10 @@ void launch_ss(hipStream_t s, const Arena &A, int n_pairs, char *ssp, char *ssn,
 void launch_anchors(hipStream_t s, const Arena &A, const int *pairs, int n,
                     const SeedDesc *frag_seeds, const int *frag_off, const int *frag_cnt,
                     uint8_t *anchor_masks, const long long *anchor_off, int *anchor_cnt,
-                    int *scratch_i, const long long *scratch_off) {
+                    int *scratch_i, const long long *scratch_off, int max_n) {
     if (n <= 0) return;
-    anchor_kernel<<<n, 256, 0, s>>>(A, pairs, frag_seeds, frag_off, frag_cnt, anchor_masks,
+    const size_t lds = (size_t)3 * sizeof(double) * (size_t)min(max_n, kLdsCand);
+    anchor_kernel<<<n, 256, lds, s>>>(A, pairs, frag_seeds, frag_off, frag_cnt, anchor_masks,
                                     anchor_off, anchor_cnt, scratch_i, scratch_off);
 }
 
