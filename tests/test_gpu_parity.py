@@ -140,6 +140,18 @@ def test_selection_cache_boundaries_vs_oracle():
     assert [int(p.valid.sum()) for p in pairs[:3]] == [511, 512, 513]
 
 
+def test_anchor_global_path_vs_oracle():
+    """A masked N = 1600 pair, beyond the 1536 residues whose anchor-seed
+    counts, candidate list, neighbourhood mask and coordinates the anchor
+    kernel keeps in LDS: its global-memory path (149 same-index valid
+    residues, 485 / 517 pred / native valid) against the oracle."""
+    _gpu_or_skip()
+    p = synth.make_pair(41600, n=1600, sigma=3.0, drop_frac=0.7)
+    assert len(p.valid) == 1600 and int(p.valid.sum()) == 149
+    s, R, t = ia.tmscore_search_batch([p])
+    _check(s, R, t, libs.oracle_search([p], serial_bank=1), "anchor-global")
+
+
 def test_reference_toy_pair():
     """The 5-residue toy of the reference tests (tests/test_gpu_scoring_refactor.py:179-190):
     the reference C path scores it 0.77317584653223026 for dp_iter 0 and 1."""
