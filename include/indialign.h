@@ -61,8 +61,14 @@ void indialign_default_config(NativeConfig *cfg);
  * Returns -1 when length <= 0 (as the reference), -2 when no usable GPU is
  * present or a device error occurred (message on stderr; the reference
  * never had this case because it fell back to its CPU path), else 0.
- * The main seed bank follows the reference's multithreaded single-pair
- * semantics (no early termination, search.h:100-136). */
+ * The main seed bank follows the reference's per-call switch
+ * (search.h:14-24, 83-136): early termination at early_term_score (its
+ * serial path) when the reference's OpenMP runtime would run one thread or
+ * the bank has fewer than 4 candidates (K seeds x C cutoffs) per thread,
+ * otherwise none (its multithreaded path).  The thread count is the one
+ * libgomp's omp_get_max_threads() gives a caller with default settings: the
+ * first entry of OMP_NUM_THREADS, else the CPUs of the process's affinity
+ * mask (a caller that changed it with omp_set_num_threads is not seen). */
 int indialign_tmscore_search(const NativePairInput *inp, const NativeConfig *cfg,
                              NativePairResult *out);
 
@@ -100,19 +106,36 @@ int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const Native
 
 /* Resident batch: copy n_pairs inputs into device memory once
  * (indialign_batch_load), then search them any number of times without
- * host-to-device traffic (indialign_batch_run; outputs has n_pairs slots,
- * same conventions as indialign_search_ex).  A later load or search call
- * replaces the resident batch.  Return 0, or -2 on a device error. */
+ * host-to-device traffic (indialign_batch_run; n_pairs must be the loaded
+ * count and outputs has n_pairs slots; same conventions as
+ * indialign_search_ex).  Any other search call (or indialign_release)
+ * replaces the resident batch; indialign_batch_run then returns -5 until the
+ * next load, as it does for an n_pairs other than the loaded count.
+ * Return 0, -1 for n_pairs < 0 (load), -2 on a device error, -3 / -4 as
+ * indialign_search_ex. */
 int indialign_batch_load(int n_pairs, const NativePairInput *inputs);
-int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int serial_bank,
-                        int method);
+int indialign_batch_run(int n_pairs, const NativeConfig *cfg, NativePairResult *outputs,
+                        int serial_bank, int method);
 
-/* Work and time of the last search.  Fills up to n
+/* Free all device memory the library holds (the engine is created again by
+ * the next call; the ranker, its budgets and the launch shape are kept).
+ * Returns 0. */
+int indialign_release(void);
+
+/* Work and time of the last search.  Fills up to n (at most 11)
  * doubles:
  *   [0] wall ms  [1] candidates refined  [2] NW cells filled
  *   [3] chains run  [4] kernel launches
  *   [5] chain-kernel ms  [6] bank-kernel ms  (device time, HIP events)
- *   [7] local-fragment passes that fell back to every work item (method 2)
+ *   [7] local-fragment passes that fell back to every work item (method 2,
+ *       margin rule)
+ *   [8] local-fragment passes that took the fallback budget (method 2)
+ *   [9] seed-bank refinement passes (iterations of the refine loop,
+ *       search.h:40-55, counted on the device)
+ *  [10] seed-bank algorithmic FP64 operations, counted on the device: per
+ *       pass 26 n (transform + distance) + 31 n_sel + 3400 (centroids,
+ *       covariance, key matrix, Jacobi solve) when n_sel >= 3, the initial
+ *       Kabsch 31 n_seed + 3400, and 29 n per TM-score (n the bank's points)
  * Returns the number of values written. */
 int indialign_last_stats(double *out, int n);
 
@@ -124,6 +147,21 @@ int indialign_last_stats(double *out, int n);
  * n_pairs x 10 doubles, row-major by pair.  Returns the number of pairs
  * written. */
 int indialign_last_stage_scores(double *out, int n_pairs);
+
+/* The Needleman-Wunsch fill and traceback of the search's NW chain kernel
+ * (nw.h:10-68 semantics) on n supplied score matrices, for tests: matrix k
+ * is plen[k] x nlen[k] (1..32767), row-major, concatenated in smat; gap[k]
+ * is its gap open.  Outputs, concatenated per matrix: H and T
+ * ((plen + 1) x (nlen + 1) each, the reference's nw_dp layout: boundary H 0
+ * and trace -1, interior trace 0 diag / 1 up / 2 left) and the aligned
+ * pairs in increasing order (min(plen, nlen) slots per matrix in ap / an,
+ * count in na[k]).  fast = 1 runs the FP32 fill of the fast method;
+ * trace_global = 1 keeps the trace in a global slot (as chains too long for
+ * LDS do), 0 in LDS.  Returns 0, -1 for bad sizes (or LDS too small for the
+ * requested placement), -2 without a GPU or on a device error. */
+int indialign_nw_matrix(int n, const double *smat, const int *plen, const int *nlen,
+                        const double *gap, int fast, int trace_global, double *H, int8_t *T,
+                        int *ap, int *an, int *na);
 
 /* The rotation solvers of the search, run on the host (the device code is
  * the same source): from the 3x3 covariance h[3k+d] = sum (p_k - cp_k)
@@ -169,6 +207,24 @@ int indialign_set_ranker(const float *params);
  * Returns 0, or -1 for a budget out of range. */
 int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets,
                                   int top_offset_banks, int top_seeds, double fallback_margin);
+
+/* Further options of method 2 (defaults 0, 512, 64):
+ *   reduce_rescored = 1 makes every local-fragment and threading-DP pass
+ *     keep the work item whose transform has the best same-index TM-score
+ *     (the score the pipeline keeps), 0 the plain search's rule: the item
+ *     with the best chain score, rescored afterwards (rescue.h:98-116,
+ *     indialign.cpp:129-152);
+ *   fallback_top_m, fallback_top_m_first: the fallback budgets.  A pair
+ *     whose largest predicted refined score of the dense (first)
+ *     local-fragment pass is below rescue_score_2 (rescue_score_1), i.e.
+ *     the ranker predicts the pass will not lift it over the threshold at
+ *     which the plain search keeps searching, refines its top
+ *     max(top_m, fallback_top_m) (max(top_m_first, fallback_top_m_first))
+ *     items instead of top_m (top_m_first): a bounded form of falling back
+ *     to the plain search's full pass.  0 disables it.
+ * Returns 0, or -1 for a value out of range. */
+int indialign_set_learned_options(int reduce_rescored, int fallback_top_m,
+                                  int fallback_top_m_first);
 
 /* Launch shape of the NW chain kernel and the seed-bank kernel (no
  * counterpart in the reference: its CPU loops have no launch shape).  For

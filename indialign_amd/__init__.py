@@ -80,9 +80,14 @@ def load_library(path: str | None = None):
     lib.indialign_cuda_available.restype = ctypes.c_int
     lib.indialign_batch_load.argtypes = [ctypes.c_int, ctypes.POINTER(NativePairInput)]
     lib.indialign_batch_load.restype = ctypes.c_int
-    lib.indialign_batch_run.argtypes = [ctypes.POINTER(NativeConfig), ctypes.POINTER(NativePairResult),
-                                        ctypes.c_int, ctypes.c_int]
+    lib.indialign_batch_run.argtypes = [ctypes.c_int, ctypes.POINTER(NativeConfig),
+                                        ctypes.POINTER(NativePairResult), ctypes.c_int,
+                                        ctypes.c_int]
     lib.indialign_batch_run.restype = ctypes.c_int
+    lib.indialign_release.argtypes = []
+    lib.indialign_release.restype = ctypes.c_int
+    lib.indialign_set_learned_options.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.indialign_set_learned_options.restype = ctypes.c_int
     lib.indialign_last_stats.argtypes = [c_double_p, ctypes.c_int]
     lib.indialign_last_stats.restype = ctypes.c_int
     c_float_p = ctypes.POINTER(ctypes.c_float)
@@ -223,21 +228,44 @@ def batch_load(pairs) -> int:
 
 
 def batch_run(cfg: NativeConfig | None = None, serial_bank: int = 1, method: int = METHOD_PLAIN):
-    """Search the resident batch (indialign_batch_run); (scores, R, t)."""
+    """Search the resident batch (indialign_batch_run); (scores, R, t).
+    Raises if another search replaced the resident batch since batch_load."""
     cfg = cfg or default_config()
     res = (NativePairResult * max(1, _resident_n))()
-    rc = load_library().indialign_batch_run(ctypes.byref(cfg), res, int(serial_bank), int(method))
+    rc = load_library().indialign_batch_run(_resident_n, ctypes.byref(cfg), res, int(serial_bank),
+                                            int(method))
+    if rc == -5:
+        raise IndialignError("indialign_batch_run: no resident batch of this size (another "
+                             "search replaced it; call batch_load again)")
     if rc != 0:
         raise IndialignError(f"indialign_batch_run returned {rc}")
     return _results(res, _resident_n)
 
 
+def release() -> None:
+    """Free the library's device memory (indialign_release)."""
+    load_library().indialign_release()
+
+
+def set_learned_options(reduce_rescored: bool = False, fallback_top_m: int = 512,
+                        fallback_top_m_first: int = 64) -> None:
+    """Method 2's options (indialign_set_learned_options): the pass reduction
+    (keep the work item with the best same-index TM-score, True, or the plain
+    search's best chain score, False) and the fallback budgets of pairs whose
+    predicted best stays below the pass's rescue threshold (0: off)."""
+    rc = load_library().indialign_set_learned_options(int(bool(reduce_rescored)),
+                                                      int(fallback_top_m),
+                                                      int(fallback_top_m_first))
+    if rc != 0:
+        raise IndialignError(f"indialign_set_learned_options returned {rc}")
+
+
 def last_stats() -> dict:
     """Work counters of the last call (see indialign_last_stats in the header)."""
-    buf = (ctypes.c_double * 8)()
-    load_library().indialign_last_stats(buf, 8)
+    buf = (ctypes.c_double * 11)()
+    load_library().indialign_last_stats(buf, 11)
     keys = ["wall_ms", "candidates", "nw_cells", "chains", "launches", "chain_ms", "bank_ms",
-            "fallbacks"]
+            "fallbacks", "fallback_budget_passes", "refine_passes", "bank_ops"]
     return {k: buf[i] for i, k in enumerate(keys)}
 
 

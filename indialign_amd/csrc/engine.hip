@@ -12,6 +12,7 @@
 #include "mlp.h"
 
 #include <dlfcn.h>
+#include <sched.h>
 
 #include <algorithm>
 #include <chrono>
@@ -94,6 +95,8 @@ constexpr double kNever = 1e300;  // early termination disabled
 
 }  // namespace
 
+constexpr int kNStats = 11;
+
 class Engine {
   public:
     Engine() {
@@ -111,11 +114,6 @@ class Engine {
         (void)hipStreamDestroy(s_);
     }
 
-    int run(const NativePairInput *inp, int n, const NativeConfig *cfg, bool serial_bank,
-            NativePairResult *out) {
-        load(inp, n);
-        return search(cfg, serial_bank, out);
-    }
     // Copy a batch into device memory (kept resident until the next load).
     void load(const NativePairInput *inp, int n);
     // Search the resident batch; out has one slot per loaded pair.
@@ -123,9 +121,15 @@ class Engine {
     // method 1: fast (FP32 NW fill, closed-form FP64 rotations).
     // method 2: learned (method 1 + only the ranker's top-m local-fragment
     //           work items of each pair are refined).
-    int search(const NativeConfig *cfg, bool serial_bank, NativePairResult *out, int method = 0);
+    // serial_bank: 1 = early termination of the main seed bank applies (the
+    // reference's serial path), 0 = it never applies (its OpenMP path);
+    // -1 = per pair, the reference's gate (search.h:14-24) with
+    // omp_threads host threads: the serial path when omp_threads <= 1 or
+    // the bank's K*C candidates are fewer than 4 per thread.
+    int search(const NativeConfig *cfg, int serial_bank, NativePairResult *out, int method = 0,
+               int omp_threads = 1);
     int loaded() const { return n_loaded_; }
-    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double stats[kNStats] = {};
     // Running best score of every loaded pair after each of kStages stages
     // (NaN: the stage did not run), row-major [pair][stage].
     static constexpr int kStages = 10;
@@ -138,6 +142,16 @@ class Engine {
     int top_offsets = 16;           // threading-DP offsets per pair that start chains
     int top_offset_banks = 16;      // threading-DP offsets per pair that run their bank
     int top_seeds = 0;              // seeds refined per SS-stage and threading bank (0 = all)
+    // Fallback budgets: a pair whose largest predicted refined score of a
+    // local-fragment pass is below that pass's rescue threshold (the case
+    // where the plain search is the fallback) refines this many items.
+    int fallback_top_m = 512;        // dense pass (below rescue_score_2)
+    int fallback_top_m_first = 64;   // first pass (below rescue_score_1)
+    // Method 2: a rescue pass keeps its work item whose transform has the
+    // best same-index TM-score (true) or, as the plain search, the item with
+    // the best chain score, rescored afterwards (false, the default: it keeps
+    // the learned search on the plain search's trajectory).
+    bool reduce_rescored = false;
     // Chain-kernel launch shape (indialign_set_launch_tuning): wavefronts per
     // SIMD of the kernel variant (2 or 3), persistent slots per CU (0: as many
     // as are resident), NW trace placement (0: LDS only when that costs no
@@ -164,7 +178,8 @@ class Engine {
     hipEvent_t e0_, e1_;
     int n_cu_ = 256;
     const NativeConfig *cfg_ = nullptr;
-    bool serial_ = true;
+    int serial_ = 1;
+    int omp_threads_ = 1;
     bool fast_ = false;
     bool learned_ = false;
     std::vector<HostPair> hp_;
@@ -217,7 +232,7 @@ class Engine {
         b_red_cnt, b_red_out, b_rs_pair, b_rs_in, b_rs_out, b_pairs, b_aoff, b_acnt, b_ascr,
         b_soff, b_fofs, b_fcnt, b_moff, b_feat, b_fgroups, b_pred, b_isel, b_srow, b_sitems,
         b_ssel, b_scnt, b_itemy, b_pmax, b_cont_cnt, b_cont_idx, b_cont_state, b_cblk_bank,
-        b_cblk_first;
+        b_cblk_first, b_rr_pair, b_rr_out, b_bank_work;
 
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
@@ -230,7 +245,7 @@ class Engine {
                 &b_rs_pair, &b_rs_in, &b_rs_out, &b_pairs, &b_aoff, &b_acnt, &b_ascr, &b_soff,
                 &b_fofs, &b_fcnt, &b_moff, &b_feat, &b_fgroups, &b_pred, &b_isel, &b_srow,
                 &b_sitems, &b_ssel, &b_scnt, &b_itemy, &b_pmax, &b_cont_cnt, &b_cont_idx,
-                &b_cont_state, &b_cblk_bank, &b_cblk_first};
+                &b_cont_state, &b_cblk_bank, &b_cblk_first, &b_rr_pair, &b_rr_out, &b_bank_work};
     }
 
     template <class T>
@@ -271,6 +286,12 @@ class Engine {
     std::vector<Hit> rescore(const std::vector<int> &pairs, const std::vector<Hit> &in);
     std::vector<Hit> group_reduce(const std::vector<long long> &offs,
                                  const std::vector<int> &cnts, const Hit *in_dev);
+    // Per group, the entry with the best same-index TM-score (groups with no
+    // entry: the rescored (kNeg, identity) of the plain path); owner[k] is
+    // the pair of group k.
+    std::vector<Hit> reduce_rescored_groups(const std::vector<long long> &offs,
+                                            const std::vector<int> &cnts, const Hit *in_dev,
+                                            const std::vector<int> &owner);
     // dp_refine (align.h:149-159) for a list of (pair, start); lite selects the
     // fused one-launch path, non-lite the per-iteration list + bank path.
     std::vector<Hit> dp_refine(const std::vector<int> &pairs, const std::vector<Hit> &start,
@@ -398,6 +419,7 @@ void Engine::prepare(const NativePairInput *inp, int n) {
     A_.ssp = b_ssp.get<char>(std::max(off, 1));
     A_.ssn = b_ssn.get<char>(std::max(off, 1));
     A_.info = upload(b_info, info);
+    A_.bank_work = b_bank_work.get<unsigned long long>(2);
     IA_CHECK(hipStreamSynchronize(s_));
 }
 
@@ -696,6 +718,38 @@ std::vector<Hit> Engine::group_reduce(const std::vector<long long> &offs,
     return download(dout, offs.size());
 }
 
+std::vector<Hit> Engine::reduce_rescored_groups(const std::vector<long long> &offs,
+                                                const std::vector<int> &cnts, const Hit *in_dev,
+                                                const std::vector<int> &owner) {
+    if (offs.empty()) return {};
+    long long n = 0;
+    for (size_t k = 0; k < offs.size(); k++) n = std::max(n, offs[k] + cnts[k]);
+    std::vector<int> pair_of((size_t)n, owner[0]);
+    for (size_t k = 0; k < offs.size(); k++)
+        for (int j = 0; j < cnts[k]; j++) pair_of[(size_t)(offs[k] + j)] = owner[k];
+    Hit *rs = b_rr_out.get<Hit>((size_t)std::max(n, 1LL));
+    if (n > 0) {
+        launch_rescore(s_, A_, upload(b_rr_pair, pair_of), in_dev, (int)n, rs, true);
+        launched();
+        IA_CHECK(hipGetLastError());
+    }
+    std::vector<Hit> red = group_reduce(offs, cnts, rs);
+    std::vector<int> ep;
+    std::vector<Hit> eh;
+    std::vector<size_t> ek;
+    for (size_t k = 0; k < red.size(); k++)
+        if (cnts[k] == 0) {
+            ep.push_back(owner[k]);
+            eh.push_back(red[k]);
+            ek.push_back(k);
+        }
+    if (!ep.empty()) {
+        const std::vector<Hit> r = rescore(ep, eh);
+        for (size_t i = 0; i < ek.size(); i++) red[ek[i]] = r[i];
+    }
+    return red;
+}
+
 std::vector<Hit> Engine::dp_refine(const std::vector<int> &pairs, const std::vector<Hit> &start,
                                    int iters, bool lite, int screen) {
     const size_t n = pairs.size();
@@ -868,7 +922,9 @@ void Engine::stage_seed_bank() {
         b.init = INIT_SEED;
         b.cand_cap = K * h.info.C;
         b.out_idx = a;
-        b.et = serial_ ? cfg_->early_term_score : kNever;
+        const bool serial = serial_ >= 0 ? serial_ != 0
+                                          : !(omp_threads_ > 1 && K * h.info.C >= 4 * omp_threads_);
+        b.et = serial ? cfg_->early_term_score : kNever;
         b.sd8 = h.info.sd8;
         std::memcpy(b.cut, h.info.cut, sizeof(b.cut));
         banks.push_back(b);
@@ -1083,8 +1139,13 @@ void Engine::stage_rescue() {
             launched();
         }
         if (rank) {
-            // refine only the ranker's top-m work items of every pair
+            // refine only the ranker's top-m work items of every pair; a pair
+            // whose predicted best stays below this pass's rescue threshold
+            // (where the plain search would be the fallback) gets the wider
+            // fallback budget
             const int top_m = dense ? this->top_m : top_m_first;
+            const int fb_m = dense ? fallback_top_m : fallback_top_m_first;
+            const int m_cap = std::max(top_m, fb_m);
             float *pred = b_pred.get<float>(rows);
             ranker.forward(s_, X, rows, pred);
             std::vector<long long> row0(fgs.size()), sel_off(fgs.size());
@@ -1094,16 +1155,18 @@ void Engine::stage_rescue() {
                 row0[k] = fgs[k].row0;
                 items[k] = fgs[k].items;
                 sel_off[k] = ns;
-                ns += std::min(fgs[k].items, top_m);
+                ns += std::min(fgs[k].items, m_cap);
             }
             int *isel = b_isel.get<int>(ns);
             float *pmax_d = b_pmax.get<float>(fgs.size());
+            int *cnt_d = b_scnt.get<int>(fgs.size());
             launch_select_top(s_, pred, upload(b_srow, row0), upload(b_sitems, items),
-                              (int)fgs.size(), top_m, isel, upload(b_ssel, sel_off),
-                              b_scnt.get<int>(fgs.size()), pmax_d);
+                              (int)fgs.size(), top_m, isel, upload(b_ssel, sel_off), cnt_d, pmax_d,
+                              fb_m, (float)(dense ? rs2 : rs1));
             launched();
-            // fallback: a pair whose best predicted refined score clears its
-            // running best by the margin refines every work item
+            const std::vector<int> sel_cnt = download(cnt_d, fgs.size());
+            // margin fallback: a pair whose best predicted refined score
+            // clears its running best by the margin refines every work item
             std::vector<float> pmax;
             if (fallback_margin >= 0) pmax = download(pmax_d, fgs.size());
             A_.isel = isel;
@@ -1116,7 +1179,7 @@ void Engine::stage_rescue() {
                     ChainGroup &g = groups[k];
                     const bool full = fallback_margin >= 0 &&
                                       (double)pmax[k] >= best_[a].score + fallback_margin;
-                    const int c = full ? fgs[k].items : std::min(fgs[k].items, top_m);
+                    const int c = full ? fgs[k].items : sel_cnt[k];
                     g.sel = full ? 0 : 1;
                     g.sel_off = sel_off[k];
                     g.n_chains = 2 * c;
@@ -1125,6 +1188,7 @@ void Engine::stage_rescue() {
                     cnts.push_back(2 * c);
                     ob += 2LL * c;
                     stats[7] += full ? 1 : 0;
+                    stats[8] += (!full && c > std::min(fgs[k].items, top_m)) ? 1 : 0;
                     k++;
                 } else {
                     offs.push_back(ob);
@@ -1151,8 +1215,9 @@ void Engine::stage_rescue() {
                 rec->n += take;
             }
         }
-        const std::vector<Hit> red = group_reduce(offs, cnts, dout);
-        const std::vector<Hit> rs = rescore(P, red);
+        const std::vector<Hit> rs = learned_ && reduce_rescored
+                                        ? reduce_rescored_groups(offs, cnts, dout, P)
+                                        : rescore(P, group_reduce(offs, cnts, dout));
         for (size_t k = 0; k < P.size(); k++) offer(P[k], rs[k]);
         profile(name, t0);
     };
@@ -1291,8 +1356,9 @@ void Engine::stage_rescue() {
                 }
             }
             run_chains(groups, dthr, dthr);
-            const std::vector<Hit> red = group_reduce(offs, cnts, dthr);
-            const std::vector<Hit> rs = rescore(P, red);
+            const std::vector<Hit> rs = learned_ && reduce_rescored
+                                            ? reduce_rescored_groups(offs, cnts, dthr, P)
+                                            : rescore(P, group_reduce(offs, cnts, dthr));
             for (size_t k = 0; k < P.size(); k++) offer(P[k], rs[k]);
             profile("threading_dp", t0);
         }
@@ -1479,12 +1545,13 @@ void Engine::load(const NativePairInput *inp, int n) {
     prepare(inp, n);
 }
 
-int Engine::search(const NativeConfig *cfg, bool serial_bank, NativePairResult *out,
-                   int method) {
+int Engine::search(const NativeConfig *cfg, int serial_bank, NativePairResult *out,
+                   int method, int omp_threads) {
     const auto t0 = std::chrono::steady_clock::now();
     for (double &x : stats) x = 0;
     cfg_ = cfg;
     serial_ = serial_bank;
+    omp_threads_ = omp_threads;
     fast_ = method == 1 || method == 2;
     learned_ = method == 2;
     if (learned_ && !ranker.loaded())
@@ -1499,6 +1566,7 @@ int Engine::search(const NativeConfig *cfg, bool serial_bank, NativePairResult *
         set_identity(out[k].rotation, out[k].translation);
     }
     if (!hp_.empty()) {
+        IA_CHECK(hipMemsetAsync(A_.bank_work, 0, 2 * sizeof(unsigned long long), s_));
         enumerate_seeds();
         stage_seed_bank();
         stage_ss();
@@ -1507,6 +1575,9 @@ int Engine::search(const NativeConfig *cfg, bool serial_bank, NativePairResult *
         stage_topk();
         stage_dp();
         stage_weighted();
+        const std::vector<unsigned long long> bw = download(A_.bank_work, 2);
+        stats[9] = (double)bw[1];
+        stats[10] = (double)bw[0];
         for (size_t a = 0; a < hp_.size(); a++) {
             NativePairResult &o = out[hp_[a].src];
             o.score = best_[a].score;
@@ -1524,7 +1595,7 @@ int Engine::search(const NativeConfig *cfg, bool serial_bank, NativePairResult *
 namespace {
 std::mutex g_mu;
 std::unique_ptr<ia::Engine> g_engine;
-double g_stats[8];
+double g_stats[ia::kNStats];
 // Ranker parameters installed by indialign_set_ranker (host copy, so that an
 // engine re-created after a device error gets them again).
 std::vector<float> g_ranker;
@@ -1590,23 +1661,44 @@ void load_tuning_file() {
     }
 }
 
-// With g_mu held and g_engine alive: install the current launch shape.
-void sync_tuning() {
+// With g_mu held: the engine, created on first use.  A new engine (also
+// after a device error dropped the old one) has no ranker, budgets or launch
+// shape yet; sync_engine installs them.
+bool g_engine_synced = false;
+// The loaded batch is the caller's resident batch (indialign_batch_load) and
+// no other call has replaced it since.
+bool g_resident = false;
+int g_resident_n = 0;
+bool g_learned_rescored = false;
+int g_fallback_m = 512;
+int g_fallback_m_first = 64;
+
+ia::Engine &engine() {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        throw ia::DevError("no HIP device available");
+    if (!g_engine) {
+        g_engine.reset(new ia::Engine());
+        g_engine_synced = false;
+    }
+    return *g_engine;
+}
+
+void drop_engine() {
+    g_engine.reset();
+    g_engine_synced = false;
+    g_resident = false;
+}
+
+// With g_mu held and g_engine alive: install the current ranker, budgets and
+// launch shape.
+void sync_engine() {
     load_tuning_file();
     g_engine->chain_waves = g_tune[0];
     g_engine->chain_slots_per_cu = g_tune[1];
     g_engine->chain_trace = g_tune[2];
     g_engine->bank_pass_cap = g_tune[3];
-}
-
-// With g_mu held and g_engine alive: install the current ranker.
-void sync_ranker() {
-    static ia::Engine *seen = nullptr;
-    if (seen != g_engine.get()) {
-        seen = g_engine.get();
-        g_engine_ranker_ver = 0;
-    }
-    if (g_engine_ranker_ver == g_ranker_ver) return;
+    if (g_engine_synced && g_engine_ranker_ver == g_ranker_ver) return;
     if (g_ranker.empty()) g_engine->ranker.unload();
     else g_engine->ranker.set_params(g_engine->stream(), g_ranker.data());
     g_engine->top_m = g_top_m;
@@ -1615,7 +1707,29 @@ void sync_ranker() {
     g_engine->top_offsets = g_top_offsets;
     g_engine->top_offset_banks = g_top_offset_banks;
     g_engine->top_seeds = g_top_seeds;
+    g_engine->reduce_rescored = g_learned_rescored;
+    g_engine->fallback_top_m = g_fallback_m;
+    g_engine->fallback_top_m_first = g_fallback_m_first;
     g_engine_ranker_ver = g_ranker_ver;
+    g_engine_synced = true;
+}
+
+// Host threads the reference's OpenMP runtime would use for a call from the
+// caller's thread (omp_get_max_threads with default settings): the first
+// entry of OMP_NUM_THREADS, else the CPUs of this process's affinity mask.
+int host_omp_threads() {
+    const char *env = std::getenv("OMP_NUM_THREADS");
+    if (env && *env) {
+        const int v = std::atoi(env);
+        if (v > 0) return v;
+    }
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof set, &set) == 0) {
+        const int c = CPU_COUNT(&set);
+        if (c > 0) return c;
+    }
+    return 1;
 }
 
 // Standalone ranker for training and direct evaluation (own stream).
@@ -1641,25 +1755,20 @@ Trainer &trainer() {
 }
 
 int run_search(int n, const NativePairInput *inp, const NativeConfig *cfg, NativePairResult *out,
-               int serial, int method = 0) {
-    const char *env = std::getenv("INDIALIGN_SERIAL_BANK");
-    if (env && *env) serial = std::atoi(env) != 0;
+               int serial, int method = 0, int omp_threads = 1) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (method == 2 && g_ranker.empty()) return -4;
     try {
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-            throw ia::DevError("no HIP device available");
-        if (!g_engine) g_engine.reset(new ia::Engine());
-        sync_ranker();
-        sync_tuning();
-        g_engine->load(inp, n);
-        const int rc = g_engine->search(cfg, serial != 0, out, method);
-        std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
+        ia::Engine &E = engine();
+        sync_engine();
+        g_resident = false;
+        E.load(inp, n);
+        const int rc = E.search(cfg, serial, out, method, omp_threads);
+        std::memcpy(g_stats, E.stats, sizeof(g_stats));
         return rc;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "indialign_amd: %s\n", e.what());
-        g_engine.reset();
+        drop_engine();
         return -2;
     }
 }
@@ -1682,7 +1791,7 @@ void indialign_default_config(NativeConfig *cfg) {
 int indialign_tmscore_search(const NativePairInput *inp, const NativeConfig *cfg,
                              NativePairResult *out) {
     if (inp->length <= 0) return -1;
-    return run_search(1, inp, cfg, out, 0);
+    return run_search(1, inp, cfg, out, -1, 0, host_omp_threads());
 }
 
 int indialign_tmscore_search_batch(int n_pairs, const NativePairInput *inputs,
@@ -1695,42 +1804,107 @@ int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const Native
                         NativePairResult *outputs, int serial_bank, int method) {
     if (n_pairs <= 0) return 0;
     if (method < 0 || method > 2) return -3;
-    return run_search(n_pairs, inputs, cfg, outputs, serial_bank, method);
+    return run_search(n_pairs, inputs, cfg, outputs, serial_bank != 0 ? 1 : 0, method);
 }
 
 int indialign_batch_load(int n_pairs, const NativePairInput *inputs) {
+    if (n_pairs < 0) return -1;
     std::lock_guard<std::mutex> lk(g_mu);
     try {
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-            throw ia::DevError("no HIP device available");
-        if (!g_engine) g_engine.reset(new ia::Engine());
-        g_engine->load(inputs, n_pairs < 0 ? 0 : n_pairs);
+        ia::Engine &E = engine();
+        g_resident = false;
+        E.load(inputs, n_pairs);
+        g_resident = true;
+        g_resident_n = n_pairs;
         return 0;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "indialign_amd: %s\n", e.what());
-        g_engine.reset();
+        drop_engine();
         return -2;
     }
 }
 
-int indialign_batch_run(const NativeConfig *cfg, NativePairResult *outputs, int serial_bank,
-                        int method) {
+int indialign_batch_run(int n_pairs, const NativeConfig *cfg, NativePairResult *outputs,
+                        int serial_bank, int method) {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (!g_engine) return -2;
     if (method < 0 || method > 2) return -3;
     if (method == 2 && g_ranker.empty()) return -4;
+    if (!g_engine || !g_resident || n_pairs != g_resident_n) return -5;
     try {
-        sync_ranker();
-        sync_tuning();
-        const int rc = g_engine->search(cfg, serial_bank != 0, outputs, method);
+        sync_engine();
+        const int rc = g_engine->search(cfg, serial_bank != 0 ? 1 : 0, outputs, method);
         std::memcpy(g_stats, g_engine->stats, sizeof(g_stats));
         return rc;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "indialign_amd: %s\n", e.what());
-        g_engine.reset();
+        drop_engine();
         return -2;
     }
+}
+
+int indialign_release(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    drop_engine();
+    return 0;
+}
+
+int indialign_nw_matrix(int n, const double *smat, const int *plen, const int *nlen,
+                        const double *gap, int fast, int trace_global, double *H, int8_t *T,
+                        int *ap, int *an, int *na) {
+    if (n < 0 || fast < 0 || fast > 1 || trace_global < 0 || trace_global > 1) return -1;
+    if (n == 0) return 0;
+    std::vector<ia::NwProblem> pr(n);
+    long long so = 0, ho = 0, ao = 0;
+    int mp = 1, mn = 1;
+    for (int k = 0; k < n; k++) {
+        if (plen[k] < 1 || nlen[k] < 1 || plen[k] > 32767 || nlen[k] > 32767) return -1;
+        pr[k] = ia::NwProblem{so, ho, ao, plen[k], nlen[k], gap[k]};
+        so += (long long)plen[k] * nlen[k];
+        ho += (long long)(plen[k] + 1) * (nlen[k] + 1);
+        ao += std::min(plen[k], nlen[k]);
+        mp = std::max(mp, plen[k]);
+        mn = std::max(mn, nlen[k]);
+    }
+    if (ia::nw_matrix_lds_bytes(mp, mn, trace_global == 0) > 160 * 1024) return -1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return -2;
+    std::lock_guard<std::mutex> lk(g_tmu);
+    void *buf[8] = {};
+    hipStream_t st = nullptr;
+    int rc = 0;
+    try {
+        const long long tw = trace_global ? ia::nw_matrix_trace_words(mp, mn) * n : 0;
+        const size_t sz[8] = {sizeof(ia::NwProblem) * n, sizeof(double) * so, sizeof(double) * ho,
+                              (size_t)ho, sizeof(int) * ao, sizeof(int) * ao, sizeof(int) * n,
+                              sizeof(unsigned) * (size_t)tw};
+        for (int b = 0; b < 8; b++)
+            if (sz[b]) IA_CHECK(hipMalloc(&buf[b], sz[b]));
+        IA_CHECK(hipStreamCreate(&st));
+        IA_CHECK(hipMemcpyAsync(buf[0], pr.data(), sz[0], hipMemcpyHostToDevice, st));
+        IA_CHECK(hipMemcpyAsync(buf[1], smat, sz[1], hipMemcpyHostToDevice, st));
+        IA_CHECK(hipMemsetAsync(buf[2], 0, sz[2], st));     // H boundary 0 (nw.h:14-17)
+        IA_CHECK(hipMemsetAsync(buf[3], 0xff, sz[3], st));  // trace boundary -1
+        ia::launch_nw_matrix(st, (const ia::NwProblem *)buf[0], n, (const double *)buf[1],
+                             (double *)buf[2], (signed char *)buf[3], (int *)buf[4], (int *)buf[5],
+                             (int *)buf[6], mp, mn, trace_global ? (unsigned *)buf[7] : nullptr,
+                             fast != 0);
+        IA_CHECK(hipGetLastError());
+        IA_CHECK(hipMemcpyAsync(H, buf[2], sz[2], hipMemcpyDeviceToHost, st));
+        IA_CHECK(hipMemcpyAsync(T, buf[3], sz[3], hipMemcpyDeviceToHost, st));
+        if (ao) {
+            IA_CHECK(hipMemcpyAsync(ap, buf[4], sz[4], hipMemcpyDeviceToHost, st));
+            IA_CHECK(hipMemcpyAsync(an, buf[5], sz[5], hipMemcpyDeviceToHost, st));
+        }
+        IA_CHECK(hipMemcpyAsync(na, buf[6], sz[6], hipMemcpyDeviceToHost, st));
+        IA_CHECK(hipStreamSynchronize(st));
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "indialign_amd: %s\n", e.what());
+        rc = -2;
+    }
+    if (st) (void)hipStreamDestroy(st);
+    for (void *b : buf)
+        if (b) (void)hipFree(b);
+    return rc;
 }
 
 int indialign_cuda_available(void) {
@@ -1741,7 +1915,7 @@ int indialign_cuda_available(void) {
 
 int indialign_last_stats(double *out, int n) {
     std::lock_guard<std::mutex> lk(g_mu);
-    const int m = n < 8 ? n : 8;
+    const int m = n < ia::kNStats ? n : ia::kNStats;
     for (int k = 0; k < m; k++) out[k] = g_stats[k];
     return m;
 }
@@ -1788,6 +1962,19 @@ int indialign_set_learned_budgets(int top_m, int top_m_first, int top_offsets,
     return 0;
 }
 
+int indialign_set_learned_options(int reduce_rescored, int fallback_top_m,
+                                  int fallback_top_m_first) {
+    if (reduce_rescored < 0 || reduce_rescored > 1 || fallback_top_m < 0 ||
+        fallback_top_m_first < 0)
+        return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_learned_rescored = reduce_rescored != 0;
+    g_fallback_m = fallback_top_m;
+    g_fallback_m_first = fallback_top_m_first;
+    g_ranker_ver++;
+    return 0;
+}
+
 int indialign_set_launch_tuning(int chain_waves, int chain_slots_per_cu, int chain_trace,
                                 int bank_pass_cap) {
     if (!tune_valid(chain_waves, chain_slots_per_cu, chain_trace, bank_pass_cap)) return -1;
@@ -1813,11 +2000,9 @@ int indialign_collect_records(int n_pairs, const NativePairInput *inputs, const 
     if (n_pairs <= 0 || cap < 0) return -1;
     std::lock_guard<std::mutex> lk(g_mu);
     try {
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-            throw ia::DevError("no HIP device available");
-        if (!g_engine) g_engine.reset(new ia::Engine());
-        sync_tuning();
+        engine();
+        sync_engine();
+        g_resident = false;
         ia::Engine::RecordSink sink{X, y, pair_idx, cap, 0};
         g_engine->rec = &sink;
         g_engine->load(inputs, n_pairs);
@@ -1834,7 +2019,7 @@ int indialign_collect_records(int n_pairs, const NativePairInput *inputs, const 
         return rc;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "indialign_amd: %s\n", e.what());
-        g_engine.reset();
+        drop_engine();
         return -2;
     }
 }

@@ -60,6 +60,10 @@ struct Arena {
     const P4 *p4, *q4;                                 // full-index AoS copies (p4.w = valid)
     const P4 *pc4, *qc4;                               // compact AoS copies
     const P8 *pq8;                                     // full-index pairs (pred, native)
+    // seed-bank work counters: [0] algorithmic FP64 operations of the refine
+    // loop and TM-scores (per pass 26 n + 31 n_sel + 3400, score 29 n),
+    // [1] refinement passes
+    unsigned long long *bank_work;
 };
 
 IA_HD void set_identity(double *R, double *t) {

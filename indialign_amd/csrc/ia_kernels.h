@@ -38,7 +38,8 @@ struct BankDesc {
 // ---- NW chains (one wavefront per chain) ---------------------------------
 enum ChainInit { CI_XFORM = 0, CI_FRAG = 1, CI_OFFSET = 2 };
 enum ChainMode { CM_LITE = 0, CM_LIST = 1 };
-enum ScoreMode { SM_TM = 0, SM_SS = 1, SM_IA4 = 2 };
+// SM_MAT: score read from a supplied matrix (the NW test entry only)
+enum ScoreMode { SM_TM = 0, SM_SS = 1, SM_IA4 = 2, SM_MAT = 3 };
 
 // A group of chains of one pair that share everything but the start.
 //   CI_XFORM:  chain of item i, gap g starts from
@@ -114,8 +115,10 @@ void launch_bank_reduce(hipStream_t s, const BankDesc *banks, int n_banks, const
                         const Hit *cands, Hit *out);
 void launch_group_reduce(hipStream_t s, const long long *offs, const int *counts, int n,
                          const Hit *in, Hit *out);
+// Same-index TM-score of each entry's (R, t); keep_none leaves entries
+// without a result (score kNeg) as they are.
 void launch_rescore(hipStream_t s, const Arena &A, const int *pair_of, const Hit *in, int n,
-                    Hit *out);
+                    Hit *out, bool keep_none = false);
 void launch_topk(hipStream_t s, const Arena &A, const int *pairs, const Hit *best, int n,
                  const long long *mask_off, uint8_t *masks);
 void launch_weighted(hipStream_t s, const Arena &A, const int *pairs, const Hit *best, int n,
@@ -131,6 +134,20 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
                    int n_slots, bool cols_in_lds, bool fast, int waves);
+// Needleman-Wunsch over supplied score matrices (test entry): problem k has
+// a plen x nlen row-major matrix at smat + s_off, its H and trace codes go
+// to H / T + h_off ((plen + 1) x (nlen + 1), interior only) and its aligned
+// pairs (increasing order) to ap / an + a_off; one wavefront per problem.
+struct NwProblem {
+    long long s_off, h_off, a_off;
+    int plen, nlen;
+    double gap;
+};
+size_t nw_matrix_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds);
+long long nw_matrix_trace_words(int max_plen, int max_nlen);
+void launch_nw_matrix(hipStream_t s, const NwProblem *probs, int n, const double *smat, double *H,
+                      signed char *T, int *ap, int *an, int *na, int max_plen, int max_nlen,
+                      unsigned *trace_g, bool fast);
 // ---- learned ranking of local-fragment work items ------------------------
 constexpr int kFeat = 96;   // descriptor width
 constexpr int kHid = 256;   // hidden units per layer
@@ -194,10 +211,12 @@ void launch_adam(hipStream_t s, float *p, const float *g, float *m, float *v, lo
                  float lr, float b1, float b2, float eps, int step);
 // Per group: indices (relative to the group) of the top-m predictions, ties
 // to the lower index, written in increasing index order to
-// sel[sel_off[g] ...]; count[g] = min(m, items); pmax[g] = largest prediction.
+// sel[sel_off[g] ...]; pmax[g] = largest prediction; a group whose pmax is
+// below thr takes m_low instead of m when m_low > m; count[g] = min(that m,
+// items).
 void launch_select_top(hipStream_t s, const float *pred, const long long *row0,
                        const int *items, int n_groups, int m, int *sel, const long long *sel_off,
-                       int *count, float *pmax);
+                       int *count, float *pmax, int m_low = 0, float thr = 0.f);
 
 // out[i] = "no result" (score -inf, identity) for i < n.
 void launch_fill_none(hipStream_t s, Hit *out, long long n);

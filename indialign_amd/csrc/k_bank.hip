@@ -7,6 +7,7 @@
 // the lane, which is the order of the reference loops, so a candidate's
 // (R, t, score) equals the FP64 CPU result bit for bit.
 #include "ia_kernels.h"
+#include "wave.h"
 
 #include <math.h>
 
@@ -74,7 +75,7 @@ __shared__ unsigned sel_cache[(kSelCache / 32) * 256];
 
 template <int PTS, bool FAST>
 IA_D bool refine_passes(const Arena &A, const BankDesc &B, int n, int c, const P8 *lq, double *R,
-                        double *t, int &it, int &last, int it_end) {
+                        double *t, int &it, int &last, int it_end, int &kops) {
     const int off = A.info[B.pair].off;
     const double cut = B.cut[c];
     const double lim = cut * cut;
@@ -140,6 +141,7 @@ IA_D bool refine_passes(const Arena &A, const BankDesc &B, int n, int c, const P
                 ks.template solve<FAST>(R, t);
             }
         }
+        if (cnt >= 3) kops += 31 * cnt + 3400;  // centroid + covariance + solve
         if (cnt == last) {
             it++;
             return true;
@@ -174,10 +176,12 @@ IA_D double candidate_score(const Arena &A, const BankDesc &B, int n, const P8 *
 // TM-score of search.h:358-362.
 // Returns true with out = the finished candidate, or false when its
 // refinement stopped at pass_cap (> 0): out holds its (R, t), st the rest.
+// kops accumulates the Kabsch operations (initial and per pass); st.it
+// receives the passes run.
 template <int PTS, bool FAST>
 IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const SeedDesc *seeds,
                         const uint8_t *masks, const P8 *lq, int pass_cap, Hit &out,
-                        ContState &st) {
+                        ContState &st, int &kops) {
     const PairInfo &pi = A.info[B.pair];
     const int off = pi.off;
     const int C = B.C;
@@ -211,8 +215,10 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         if (ks.cnt < (double)B.min_ali) {
             out.score = -INFINITY;
             set_identity(out.R, out.t);
+            st.it = 0;
             return true;
         }
+        kops += 31 * (int)ks.cnt + 3400;
         if (FAST) {
             ks.finish_moments();
         } else {
@@ -237,6 +243,7 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
             if (FAST) ks.add_moment(x, y, z, qa, qb, qc);
             else ks.add_centroid(x, y, z, qa, qb, qc);
         }
+        if (ks.cnt >= 3.0) kops += 31 * (int)ks.cnt + 3400;
         if (ks.cnt < 3.0) {
             set_identity(R, t);
         } else if (FAST) {
@@ -257,13 +264,14 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
     // d0s-cutoff refinement; stop when the inlier count repeats.
     int it = 0, last = -1;
     const bool done = refine_passes<PTS, FAST>(A, B, n, c, lq, R, t, it, last,
-                                               pass_cap > 0 ? min(pass_cap, B.iters) : B.iters);
+                                               pass_cap > 0 ? min(pass_cap, B.iters) : B.iters,
+                                               kops);
 #pragma unroll
     for (int q = 0; q < 9; q++) out.R[q] = R[q];
     out.t[0] = t[0]; out.t[1] = t[1]; out.t[2] = t[2];
+    st.last = last;
+    st.it = it;
     if (!done) {  // stopped at the pass cap: the continuation kernel resumes it
-        st.last = last;
-        st.it = it;
         out.score = kNeg;
         return false;
     }
@@ -288,26 +296,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void b
     const int bank = blk_bank[blockIdx.x];
     const BankDesc &B = banks[bank];
     const int j = blk_first[blockIdx.x] + (int)threadIdx.x;
-    if (j >= B.cand_cap) return;
-    Hit h;
+    const bool live = j < B.cand_cap;
     ContState st{-1, 0};
     bool done = true;
+    int kops = 0;
     const int n = bank_points(B, list_na);
-    const int K = bank_live_seeds(B, n);
-    if (j >= K * B.C || n <= 0) {
-        h.score = -INFINITY;
-        set_identity(h.R, h.t);
-    } else if (B.pts == PTS_FULL) {
-        done = run_candidate<PTS_FULL, FAST>(A, B, n, j, seeds, masks, nullptr, pass_cap, h, st);
-    } else if (B.pts == PTS_LIST) {
-        const long long base = (long long)B.a0 * list_cap;
-        done = run_candidate<PTS_LIST, FAST>(A, B, n, j, seeds, masks, list_pq + base, pass_cap, h,
-                                             st);
-    } else {
-        done = run_candidate<PTS_CWIN, FAST>(A, B, n, j, seeds, masks, nullptr, pass_cap, h, st);
+    bool ran = false;
+    if (live) {
+        Hit h;
+        const int K = bank_live_seeds(B, n);
+        if (j >= K * B.C || n <= 0) {
+            h.score = -INFINITY;
+            set_identity(h.R, h.t);
+        } else if (B.pts == PTS_FULL) {
+            ran = true;
+            done = run_candidate<PTS_FULL, FAST>(A, B, n, j, seeds, masks, nullptr, pass_cap, h, st,
+                                                 kops);
+        } else if (B.pts == PTS_LIST) {
+            ran = true;
+            const long long base = (long long)B.a0 * list_cap;
+            done = run_candidate<PTS_LIST, FAST>(A, B, n, j, seeds, masks, list_pq + base, pass_cap,
+                                                 h, st, kops);
+        } else {
+            ran = true;
+            done = run_candidate<PTS_CWIN, FAST>(A, B, n, j, seeds, masks, nullptr, pass_cap, h, st,
+                                                 kops);
+        }
+        cands[B.cand_off + j] = h;
     }
-    cands[B.cand_off + j] = h;
-    if (pass_cap <= 0) return;
+    {   // work counters: one atomic per wavefront (every lane is still here)
+        const double ops = ran ? (double)kops + 26.0 * n * st.it + (done ? 29.0 * n : 0.0) : 0.0;
+        const double passes = ran ? (double)st.it : 0.0;
+        const double so = wave_sum(ops), sp = wave_sum(passes);
+        if ((threadIdx.x & 63) == 0 && sp > 0) {
+            atomicAdd(&A.bank_work[0], (unsigned long long)so);
+            atomicAdd(&A.bank_work[1], (unsigned long long)sp);
+        }
+    }
+    if (!live || pass_cap <= 0) return;
     // append the unfinished candidates of this wavefront (one atomic per wave)
     const unsigned long long m = __ballot(!done);
     if (m == 0) return;
@@ -335,34 +361,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void b
     const int bank = blk_bank[blockIdx.x];
     const BankDesc &B = banks[bank];
     const int k = blk_first[blockIdx.x] + (int)threadIdx.x;
-    if (k >= cont_cnt[bank]) return;
-    const int j = cont_idx[B.cand_off + k];
-    Hit &h = cands[B.cand_off + j];
-    ContState st = cont_state[B.cand_off + j];
-    double R[9], t[3];
-#pragma unroll
-    for (int q = 0; q < 9; q++) R[q] = h.R[q];
-    t[0] = h.t[0]; t[1] = h.t[1]; t[2] = h.t[2];
+    const bool live = k < cont_cnt[bank];
     const int n = bank_points(B, list_na);
-    const int c = j % B.C;
-    double score;
-    if (B.pts == PTS_FULL) {
-        refine_passes<PTS_FULL, FAST>(A, B, n, c, nullptr, R, t, st.it, st.last, B.iters);
-        score = candidate_score<PTS_FULL, FAST>(A, B, n, nullptr, R, t);
-    } else if (B.pts == PTS_LIST) {
-        const P8 *lq = list_pq + (long long)B.a0 * list_cap;
-        refine_passes<PTS_LIST, FAST>(A, B, n, c, lq, R, t, st.it, st.last, B.iters);
-        score = candidate_score<PTS_LIST, FAST>(A, B, n, lq, R, t);
-    } else {
-        refine_passes<PTS_CWIN, FAST>(A, B, n, c, nullptr, R, t, st.it, st.last, B.iters);
-        score = candidate_score<PTS_CWIN, FAST>(A, B, n, nullptr, R, t);
-    }
-    Hit o;
+    int kops = 0, it0 = 0, it1 = 0;
+    if (live) {
+        const int j = cont_idx[B.cand_off + k];
+        Hit &h = cands[B.cand_off + j];
+        ContState st = cont_state[B.cand_off + j];
+        it0 = st.it;
+        double R[9], t[3];
 #pragma unroll
-    for (int q = 0; q < 9; q++) o.R[q] = R[q];
-    o.t[0] = t[0]; o.t[1] = t[1]; o.t[2] = t[2];
-    o.score = score;
-    h = o;
+        for (int q = 0; q < 9; q++) R[q] = h.R[q];
+        t[0] = h.t[0]; t[1] = h.t[1]; t[2] = h.t[2];
+        const int c = j % B.C;
+        double score;
+        if (B.pts == PTS_FULL) {
+            refine_passes<PTS_FULL, FAST>(A, B, n, c, nullptr, R, t, st.it, st.last, B.iters, kops);
+            score = candidate_score<PTS_FULL, FAST>(A, B, n, nullptr, R, t);
+        } else if (B.pts == PTS_LIST) {
+            const P8 *lq = list_pq + (long long)B.a0 * list_cap;
+            refine_passes<PTS_LIST, FAST>(A, B, n, c, lq, R, t, st.it, st.last, B.iters, kops);
+            score = candidate_score<PTS_LIST, FAST>(A, B, n, lq, R, t);
+        } else {
+            refine_passes<PTS_CWIN, FAST>(A, B, n, c, nullptr, R, t, st.it, st.last, B.iters, kops);
+            score = candidate_score<PTS_CWIN, FAST>(A, B, n, nullptr, R, t);
+        }
+        it1 = st.it;
+        Hit o;
+#pragma unroll
+        for (int q = 0; q < 9; q++) o.R[q] = R[q];
+        o.t[0] = t[0]; o.t[1] = t[1]; o.t[2] = t[2];
+        o.score = score;
+        h = o;
+    }
+    const double ops = live ? (double)kops + 26.0 * n * (it1 - it0) + 29.0 * n : 0.0;
+    const double passes = live ? (double)(it1 - it0) : 0.0;
+    const double so = wave_sum(ops), sp = wave_sum(passes);
+    if ((threadIdx.x & 63) == 0 && so > 0) {
+        atomicAdd(&A.bank_work[0], (unsigned long long)so);
+        atomicAdd(&A.bank_work[1], (unsigned long long)sp);
+    }
 }
 
 // Copy the aligned pairs of every PTS_LIST bank's traceback list into
@@ -478,14 +516,16 @@ __global__ __launch_bounds__(256) void group_reduce_kernel(const long long *offs
 }
 
 // Same-index TM-score without the d8 cut (core.h:152-166), one lane per item,
-// index-order sum.
+// index-order sum.  keep_none: entries without a result (score kNeg) stay
+// without one instead of being scored as the identity.
 __global__ __launch_bounds__(256) void rescore_kernel(Arena A, const int *pair_of,
-                                                      const Hit *in, int n, Hit *out) {
+                                                      const Hit *in, int n, Hit *out,
+                                                      int keep_none) {
     const int it = blockIdx.x * 256 + threadIdx.x;
     if (it >= n) return;
     const PairInfo &pi = A.info[pair_of[it]];
     Hit h = in[it];
-    if (h.score == -INFINITY) {  // skipped entry: nothing to rescore
+    if (h.score == -INFINITY || (keep_none && h.score <= kNeg)) {  // nothing to rescore
         out[it] = h;
         return;
     }
@@ -646,9 +686,9 @@ void launch_group_reduce(hipStream_t s, const long long *offs, const int *counts
 }
 
 void launch_rescore(hipStream_t s, const Arena &A, const int *pair_of, const Hit *in, int n,
-                    Hit *out) {
+                    Hit *out, bool keep_none) {
     if (n <= 0) return;
-    rescore_kernel<<<(n + 255) / 256, 256, 0, s>>>(A, pair_of, in, n, out);
+    rescore_kernel<<<(n + 255) / 256, 256, 0, s>>>(A, pair_of, in, n, out, keep_none ? 1 : 0);
 }
 
 void launch_topk(hipStream_t s, const Arena &A, const int *pairs, const Hit *best, int n,

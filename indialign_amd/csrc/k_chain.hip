@@ -105,9 +105,16 @@ IA_HD long long trace_words(int plen, int nlen) {
     return (long long)((fill_steps(plen, nlen) + 15) >> 4) * 64;
 }
 
+//
+// SM_MAT (test entry only): the score of cell (a, b) is smat[a * nlen + b]
+// instead of a function of the coordinates, and every cell's H is also
+// written to Hout[(a + 1) * (nlen + 1) + b + 1].
 template <int SMODE, class TR, bool CL, class V>
 IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const double *t,
-                  double gap_d, double cut8_d, double d0sq_s, ChainLds &L, TR *trace) {
+                  double gap_d, double cut8_d, double d0sq_s, ChainLds &L, TR *trace,
+                  const double *smat = nullptr, double *Hout = nullptr) {
+    constexpr bool kXYZ = SMODE == SM_TM || SMODE == SM_IA4;  // coordinates scored
+    constexpr bool kSS = SMODE == SM_SS || SMODE == SM_IA4;   // SS codes scored
     const int lane = threadIdx.x & 63;
     const int plen = pi.plen, nlen = pi.nlen, off = pi.off;
     constexpr bool F32 = sizeof(V) == 4;
@@ -135,12 +142,12 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     char nsp = 0;
     auto next_row = [&](int r) {
         const int rc = min(r, plen - 1);
-        if (SMODE != SM_SS) {
+        if (kXYZ) {
             double ex, ey, ez;
             xform(R, t, A.pcx[off + rc], A.pcy[off + rc], A.pcz[off + rc], ex, ey, ez);
             nmx = (V)ex; nmy = (V)ey; nmz = (V)ez;
         }
-        if (SMODE != SM_TM) nsp = A.ssp[off + A.pmap[off + rc]];
+        if (kSS) nsp = A.ssp[off + A.pmap[off + rc]];
     };
     next_row(lane);
     V mx = 0, my = 0, mz = 0;
@@ -157,11 +164,11 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     V cx = 0, cy = 0, cz = 0, c0x = 0, c0y = 0, c0z = 0;
     char cs = 0, c0s = 0;
     if constexpr (F32) {
-        if (SMODE != SM_SS) { c0x = colx(0); c0y = coly(0); c0z = colz(0); }
-        if (SMODE != SM_TM) c0s = cols(0);
+        if (kXYZ) { c0x = colx(0); c0y = coly(0); c0z = colz(0); }
+        if (kSS) c0s = cols(0);
     } else {  // step 0: lane 0 is on column 0, the others are idle
-        if (SMODE != SM_SS) { cx = colx(0); cy = coly(0); cz = colz(0); }
-        if (SMODE != SM_TM) cs = cols(0);
+        if (kXYZ) { cx = colx(0); cy = coly(0); cz = colz(0); }
+        if (kSS) cs = cols(0);
     }
     V rH = 0, rG = 0;
     int st = 0;
@@ -180,12 +187,12 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         const V hu = shr1_old(H, rH);
         const V gu = shr1_old(G, rG);
         if constexpr (F32) {
-            if (SMODE != SM_SS) {
+            if (kXYZ) {
                 cx = shr1_old(cx, c0x);
                 cy = shr1_old(cy, c0y);
                 cz = shr1_old(cz, c0z);
             }
-            if (SMODE != SM_TM) cs = (char)shr1_old((int)cs, (int)c0s);
+            if (kSS) cs = (char)shr1_old((int)cs, (int)c0s);
         }
         V Dc = D, Gl = G;
         if constexpr (PA) {
@@ -203,18 +210,20 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         V nx = 0, ny = 0, nz = 0;
         char ns = 0;
         if constexpr (F32) {
-            if (SMODE != SM_SS) { c0x = colx(rq); c0y = coly(rq); c0z = colz(rq); }
-            if (SMODE != SM_TM) c0s = cols(rq);
+            if (kXYZ) { c0x = colx(rq); c0y = coly(rq); c0z = colz(rq); }
+            if (kSS) c0s = cols(rq);
         } else {
             int bn;
             if constexpr (GC) bn = b + 1 >= P ? 0 : min(b + 1, nlen - 1);
             else bn = min(q + 1 - lane, nlen - 1);
-            if (SMODE != SM_SS) { nx = colx(bn); ny = coly(bn); nz = colz(bn); }
-            if (SMODE != SM_TM) ns = cols(bn);
+            if (kXYZ) { nx = colx(bn); ny = coly(bn); nz = colz(bn); }
+            if (kSS) ns = cols(bn);
         }
         const V2 rn = rowHG[rq];
         V s;
-        if (SMODE == SM_SS) {
+        if constexpr (SMODE == SM_MAT) {
+            s = act ? (V)smat[(long long)a * nlen + b] : (V)0;
+        } else if (SMODE == SM_SS) {
             s = (sp == cs) ? (V)1 : (V)0;
         } else {
             const V dx = mx - cx, dy = my - cy, dz = mz - cz;
@@ -243,6 +252,9 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         const unsigned tn = cD ? 0u : (Gl >= gu ? 2u : 1u);
         const V gn = cD ? diag + gap : mgl;
         if (lane == 63 && act) rowHG[b] = V2{hm, gn};  // to the next round's lane 0
+        if constexpr (SMODE == SM_MAT) {
+            if (act) Hout[(long long)(a + 1) * (nlen + 1) + b + 1] = (double)hm;
+        }
         tw = __builtin_amdgcn_alignbit(tn, tw, 2);
         D = hu; H = hm; G = gn;
         if constexpr (!F32) { cx = nx; cy = ny; cz = nz; cs = ns; }
@@ -290,12 +302,13 @@ IA_D int trace_word_index(int a, int b, int P, int &shift) {
 // alP/alN, reverse order).  nw.h:10-68 semantics.
 template <int SMODE, class TR, bool CL, bool FAST>
 IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const double *t,
-                  double gap, double cut8, double d0sq_s, ChainLds &L, TR *trace) {
+                  double gap, double cut8, double d0sq_s, ChainLds &L, TR *trace,
+                  const double *smat = nullptr, double *Hout = nullptr) {
     const int lane = threadIdx.x & 63;
     const int plen = pi.plen, nlen = pi.nlen;
     const int P = fill_period(nlen);
     typedef typename std::conditional<FAST, float, double>::type V;
-    nw_fill<SMODE, TR, CL, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
+    nw_fill<SMODE, TR, CL, V>(A, pi, R, t, gap, cut8, d0sq_s, L, trace, smat, Hout);
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
     __builtin_amdgcn_wave_barrier();
     // traceback (nw.h:42-68)
@@ -834,7 +847,80 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void ch
     }
 }
 
+// NW fill + traceback of one supplied score matrix per wavefront (the
+// device NW of the chains, checked bit for bit against the reference's
+// nw_dp / nw_traceback on the same matrix).
+template <bool FAST, bool GT>
+__global__ __launch_bounds__(64) void nw_matrix_kernel(const NwProblem *probs, const double *smat,
+                                                       double *H, signed char *T, int *ap,
+                                                       int *an, int *na_out, int max_plen,
+                                                       int max_nlen, unsigned *trace_g) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const LdsPlan P(max_plen, max_nlen, !GT, false);
+    ChainLds L;
+    L.rowHG = (double *)(lds + P.rowHG);
+    L.colv = (double *)(lds + P.colv);
+    L.misc = (int *)(lds + P.misc);
+    L.cols = (char *)(lds + P.cols);
+    L.alP = (short *)(lds + P.alP);
+    L.alN = (short *)(lds + P.alN);
+    L.trace = (unsigned *)(lds + P.trace);
+    const int lane = threadIdx.x & 63;
+    const NwProblem pr = probs[blockIdx.x];
+    PairInfo pi{};
+    pi.plen = pr.plen;
+    pi.nlen = pr.nlen;
+    const Arena A{};
+    double R[9], t[3];
+    set_identity(R, t);
+    typedef __attribute__((address_space(3))) unsigned lds_u32;
+    typedef typename std::conditional<GT, unsigned, lds_u32>::type TR;
+    TR *trace = GT ? (TR *)(trace_g + (long long)blockIdx.x * trace_words(max_plen, max_nlen))
+                   : (TR *)L.trace;
+    const int na = nw_align<SM_MAT, TR, false, FAST>(A, pi, R, t, pr.gap, 0.0, 1.0, L, trace,
+                                                     smat + pr.s_off, H + pr.h_off);
+    const int Pd = fill_period(pr.nlen);
+    const long long cells = (long long)pr.plen * pr.nlen;
+    for (long long c = lane; c < cells; c += 64) {
+        const int a = (int)(c / pr.nlen), b = (int)(c % pr.nlen);
+        int sh;
+        const unsigned w = trace[trace_word_index(a, b, Pd, sh)];
+        T[pr.h_off + (long long)(a + 1) * (pr.nlen + 1) + b + 1] = (signed char)((w >> sh) & 3u);
+    }
+    for (int q = lane; q < na; q += 64) {
+        ap[pr.a_off + q] = L.alP[na - 1 - q];
+        an[pr.a_off + q] = L.alN[na - 1 - q];
+    }
+    if (lane == 0) na_out[blockIdx.x] = na;
+}
+
 }  // namespace
+
+size_t nw_matrix_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds) {
+    return LdsPlan(max_plen, max_nlen, trace_in_lds, false).total;
+}
+
+long long nw_matrix_trace_words(int max_plen, int max_nlen) {
+    return trace_words(max_plen, max_nlen);
+}
+
+void launch_nw_matrix(hipStream_t s, const NwProblem *probs, int n, const double *smat, double *H,
+                      signed char *T, int *ap, int *an, int *na, int max_plen, int max_nlen,
+                      unsigned *trace_g, bool fast) {
+    if (n <= 0) return;
+    const size_t lds = nw_matrix_lds_bytes(max_plen, max_nlen, trace_g == nullptr);
+#define IA_LAUNCH_NWM(F, G)                                                                       \
+    hipLaunchKernelGGL((nw_matrix_kernel<F, G>), dim3(n), dim3(64), lds, s, probs, smat, H, T, ap, \
+                       an, na, max_plen, max_nlen, trace_g)
+    if (trace_g) {
+        if (fast) IA_LAUNCH_NWM(true, true);
+        else IA_LAUNCH_NWM(false, true);
+    } else {
+        if (fast) IA_LAUNCH_NWM(true, false);
+        else IA_LAUNCH_NWM(false, false);
+    }
+#undef IA_LAUNCH_NWM
+}
 
 int chain_blocks_per_cu(size_t lds, bool fast, int waves) {
     int nb = 0;

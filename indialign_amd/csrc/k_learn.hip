@@ -413,8 +413,11 @@ IA_D int block_excl(bool flag, int *wtot, int &total) {
     return base + below;
 }
 
+// Top m of each group by prediction; a group whose largest prediction is
+// below `thr` keeps its top m_low instead (when m_low > m).
 __global__ __launch_bounds__(256) void select_top_kernel(const float *pred, const long long *row0,
-                                                         const int *items, int m, int *sel,
+                                                         const int *items, int m_high, int m_low,
+                                                         float thr, int *sel,
                                                          const long long *sel_off, int *count,
                                                          float *pmax) {
     __shared__ unsigned hist[256];
@@ -425,14 +428,16 @@ __global__ __launch_bounds__(256) void select_top_kernel(const float *pred, cons
     const int n = items[gi];
     const float *pv = pred + row0[gi];
     int *out = sel + sel_off[gi];
+    int m = m_high;
     {   // largest prediction of the group
         float mx = -INFINITY;
         for (int i = threadIdx.x; i < n; i += 256) mx = fmaxf(mx, pv[i]);
         for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
         if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
         __syncthreads();
-        if (threadIdx.x == 0)
-            pmax[gi] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        const float gmax = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        if (threadIdx.x == 0) pmax[gi] = gmax;
+        if (gmax < thr && m_low > m) m = m_low;
     }
     if (n <= m) {
         for (int i = threadIdx.x; i < n; i += 256) out[i] = i;
@@ -614,10 +619,10 @@ void launch_adam(hipStream_t s, float *p, const float *g, float *m, float *v, lo
 
 void launch_select_top(hipStream_t s, const float *pred, const long long *row0,
                        const int *items, int n_groups, int m, int *sel, const long long *sel_off,
-                       int *count, float *pmax) {
+                       int *count, float *pmax, int m_low, float thr) {
     if (n_groups <= 0) return;
     hipLaunchKernelGGL(select_top_kernel, dim3(n_groups), dim3(256), 0, s, pred, row0, items, m,
-                       sel, sel_off, count, pmax);
+                       m_low, thr, sel, sel_off, count, pmax);
 }
 
 void launch_gather_rows(hipStream_t s, const float *src, const float *tsrc, const int *idx,

@@ -46,6 +46,10 @@ def test_default_config_and_no_compute_without_gpu():
     res = ia.NativePairResult()
     # N <= 0 returns -1 before touching the device (indialign.cpp:288)
     assert lib.indialign_tmscore_search(ctypes.byref(inp), ctypes.byref(cfg), ctypes.byref(res)) == -1
+    # no resident batch: batch_run is refused without touching the device
+    assert lib.indialign_batch_run(1, ctypes.byref(cfg), ctypes.byref(res), 1, 0) == -5
+    assert lib.indialign_set_learned_options(2, 512, 64) == -1
+    assert lib.indialign_set_learned_options(0, -1, 64) == -1
 
 
 LAYOUT_C = r"""
