@@ -266,6 +266,23 @@ int indialign_mlp_forward(const float *params, const float *X, long long n, floa
 int indialign_mlp_grad(const float *params, const float *X, const float *t, int n, float *grad,
                        float *loss);
 
+/* No counterpart in the reference: indialign_mlp_grad for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * training loops, harnesses that forbid hipMalloc).  params
+ * (INDIALIGN_MLP_NPARAM floats), X (n x INDIALIGN_MLP_NFEAT), t (n), grad
+ * (INDIALIGN_MLP_NPARAM) and loss (1 float) are device pointers; the work
+ * is enqueued on `stream` (a hipStream_t; NULL = the null stream) and the
+ * call returns without waiting.  The work buffers are a preallocated
+ * per-GPU array of about 16.8 MB (held until the process ends), so
+ * 1 <= n <= 4096; other n return -1 with a message.  Nothing in them needs
+ * clearing (every output is stored, not accumulated).  All calls on one GPU
+ * share that array: calls that overlap on a GPU (other threads, other
+ * streams) race.  The first call on a device also fills a constant slice of
+ * the array and waits for it.  Returns 0, -1 for bad arguments, -2 on a
+ * device error. */
+int indialign_mlp_grad_noalloc(const float *params, const float *X, const float *t, int n,
+                               float *grad, float *loss, void *stream);
+
 /* Adam training on the GPU (beta 0.9 / 0.999, eps 1e-8, constant lr):
  * `epochs` passes over the n rows in minibatches of `batch` rows, shuffled by
  * a seeded permutation; params is updated in place, epoch_loss (may be NULL)

@@ -11,6 +11,21 @@
 
 namespace ia {
 
+// Work buffers of the GEMM forward and backward passes for up to `rows` rows:
+// h1, h2, dh1, dh2 (rows x kHid), y, dy (rows), ones (rows values of 1.0).
+struct MlpWork {
+    float *h1, *h2, *dh1, *dh2, *y, *dy;
+    const float *ones;
+};
+
+// Forward through the GEMM sequence: y = f(X) for n rows (n <= work rows).
+void mlp_forward_gemm(hipStream_t s, const MlpWork &w, const float *P, const float *X, int n,
+                      float *y);
+// Gradient of mean (f(X) - t)^2 over n rows into G (kNParam floats) and the
+// loss into loss[0]; every output is stored, none accumulated.
+void mlp_backward(hipStream_t s, const MlpWork &w, const float *P, const float *X, const float *t,
+                  int n, float *G, float *loss);
+
 class Mlp {
 public:
     // Parameters (kNParam floats, layout of ia_kernels.h) from the host.
@@ -36,7 +51,7 @@ public:
 
 private:
     void alloc();
-    void fwd_chunk(hipStream_t s, const float *X, int n, float *y);
+    MlpWork work() const { return MlpWork{h1_, h2_, dh1_, dh2_, y_, dy_, ones_}; }
     DBuf bP_, bG_, bM_, bV_, bh1_, bh2_, bdh1_, bdh2_, bones_, by_, bdy_, bloss_, bW1b_, bW2b_;
     short *W1b_ = nullptr, *W2b_ = nullptr;
     bool wb_dirty_ = true;
