@@ -68,7 +68,11 @@ void indialign_default_config(NativeConfig *cfg);
  * otherwise none (its multithreaded path).  The thread count is the one
  * libgomp's omp_get_max_threads() gives a caller with default settings: the
  * first entry of OMP_NUM_THREADS, else the CPUs of the process's affinity
- * mask (a caller that changed it with omp_set_num_threads is not seen). */
+ * mask (a caller that changed it with omp_set_num_threads is not seen).
+ * A pair with more than 10239 native-valid residues (the NW chain kernel
+ * keeps one FP64 (H, G) pair per compact native residue in a wavefront's
+ * 160 KB of LDS) gets score NaN and the identity transform, and the call
+ * returns -6 (the reference has no such limit). */
 int indialign_tmscore_search(const NativePairInput *inp, const NativeConfig *cfg,
                              NativePairResult *out);
 
@@ -77,7 +81,9 @@ int indialign_tmscore_search(const NativePairInput *inp, const NativeConfig *cfg
  * main seed bank follows the reference's batch (serial) semantics, i.e.
  * early_term_score applies (search.h:83).  Returns 0, or -2 on a device
  * error.  Pairs with length <= 0 are left untouched (the reference ignored
- * per-pair return codes). */
+ * per-pair return codes).  Pairs over the 10239 native-valid residue limit
+ * (see indialign_tmscore_search) get score NaN and the identity while the
+ * others are searched; the call then returns -6. */
 int indialign_tmscore_search_batch(int n_pairs, const NativePairInput *inputs,
                                    const NativeConfig *cfg, NativePairResult *outputs);
 

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import warnings
 
 import numpy as np
 
@@ -189,6 +190,9 @@ def tmscore_search(pred, native, valid=None, pred_valid=None, native_valid=None,
 
 
 METHOD_PLAIN, METHOD_FAST, METHOD_LEARNED = 0, 1, 2
+# return code of a search with a pair whose compact native length exceeds
+# the NW chain kernel's limit (that pair's score is NaN, the others are set)
+TOO_LONG = -6
 
 
 def tmscore_search_batch(pairs, cfg: NativeConfig | None = None, serial_bank: int | None = None,
@@ -208,7 +212,10 @@ def tmscore_search_batch(pairs, cfg: NativeConfig | None = None, serial_bank: in
     else:
         sb = 1 if serial_bank is None else int(serial_bank)
         rc = lib.indialign_search_ex(pk.n, pk.inps, ctypes.byref(cfg), res, sb, int(method))
-    if rc != 0:
+    if rc == TOO_LONG:
+        warnings.warn("some pairs have more native-valid residues than the NW chain kernel "
+                      "holds; their scores are NaN", RuntimeWarning)
+    elif rc != 0:
         raise IndialignError(f"batch search returned {rc}")
     return _results(res, pk.n)
 

@@ -96,6 +96,7 @@ constexpr double kNever = 1e300;  // early termination disabled
 }  // namespace
 
 constexpr int kNStats = 11;
+constexpr int kTooLong = -6;  // return code: some pair exceeds chain_max_nlen()
 
 class Engine {
   public:
@@ -186,7 +187,9 @@ class Engine {
     std::vector<Hit> best_;
     std::vector<int> live_;  // pairs still in the pipeline (index into hp_)
     int n_loaded_ = 0;
-    std::vector<signed char> status_;  // per loaded pair: 0 search, 1 Lnorm<=2, -1 N<=0
+    // per loaded pair: 0 search, 1 Lnorm <= 2, -1 N <= 0, 2 more native-valid
+    // residues than one chain wavefront's LDS holds (chain_max_nlen())
+    std::vector<signed char> status_;
     Arena A_{};
     int list_cap_ = 1;
     int n_res_ = 0;  // residues of all loaded pairs
@@ -317,7 +320,7 @@ void Engine::prepare(const NativePairInput *inp, int n) {
         HostPair h{};
         h.src = k;
         h.N = I.length;
-        if (h.N <= 0 || I.lnorm <= 2.0) continue;  // handled by the caller
+        if (status_[k] != 0) continue;  // handled by the caller
         const int N = h.N;
         h.info.N = N;
         h.info.off = off;
@@ -1539,8 +1542,15 @@ void Engine::load(const NativePairInput *inp, int n) {
     n_loaded_ = n;
     status_.assign(n, 0);
     for (int k = 0; k < n; k++) {
-        if (inp[k].length <= 0) status_[k] = -1;
-        else if (inp[k].lnorm <= 2.0) status_[k] = 1;
+        if (inp[k].length <= 0) {
+            status_[k] = -1;
+        } else if (inp[k].lnorm <= 2.0) {
+            status_[k] = 1;
+        } else {
+            int nl = 0;
+            for (int i = 0; i < inp[k].length; i++) nl += inp[k].native_valid_mask[i] ? 1 : 0;
+            if (nl > chain_max_nlen()) status_[k] = 2;
+        }
     }
     prepare(inp, n);
 }
@@ -1559,11 +1569,15 @@ int Engine::search(const NativeConfig *cfg, int serial_bank, NativePairResult *o
     stage_scores.assign((size_t)n_loaded_ * kStages, std::nan(""));
     live_.clear();
     best_.assign(hp_.size(), identity_hit(kNeg));
-    // Lnorm <= 2 -> score 0, identity (indialign.cpp:47-51)
+    // Lnorm <= 2 -> score 0, identity (indialign.cpp:47-51); a pair too
+    // long for the chain kernel -> score NaN, identity, and the call returns
+    // kTooLong after searching the others
+    int rc = 0;
     for (int k = 0; k < n_loaded_; k++) {
-        if (status_[k] != 1) continue;
-        out[k].score = 0.0;
+        if (status_[k] != 1 && status_[k] != 2) continue;
+        out[k].score = status_[k] == 1 ? 0.0 : std::nan("");
         set_identity(out[k].rotation, out[k].translation);
+        if (status_[k] == 2) rc = kTooLong;
     }
     if (!hp_.empty()) {
         IA_CHECK(hipMemsetAsync(A_.bank_work, 0, 2 * sizeof(unsigned long long), s_));
@@ -1586,7 +1600,7 @@ int Engine::search(const NativeConfig *cfg, int serial_bank, NativePairResult *o
         }
     }
     stats[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return 0;
+    return rc;
 }
 
 }  // namespace ia

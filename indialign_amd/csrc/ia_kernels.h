@@ -143,6 +143,9 @@ struct NwProblem {
     int plen, nlen;
     double gap;
 };
+// Largest compact native length whose NW chain fits one wavefront's LDS
+// (the row hand-over buffer, 16 bytes per column).
+int chain_max_nlen();
 size_t nw_matrix_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds);
 long long nw_matrix_trace_words(int max_plen, int max_nlen);
 void launch_nw_matrix(hipStream_t s, const NwProblem *probs, int n, const double *smat, double *H,

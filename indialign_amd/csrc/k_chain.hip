@@ -936,6 +936,15 @@ int chain_blocks_per_cu(size_t lds, bool fast, int waves) {
 
 long long chain_trace_words(int max_plen, int max_nlen) { return trace_words(max_plen, max_nlen); }
 
+int chain_max_nlen() {
+    constexpr size_t kLdsMax = 160 * 1024;
+    // the smallest plan (trace and columns outside LDS) is 16 bytes per
+    // compact native residue plus 16
+    int n = (int)((kLdsMax - 16) / 16);
+    while (n > 1 && LdsPlan(1, n, false, false).total > kLdsMax) n--;
+    return n;
+}
+
 size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds) {
     return LdsPlan(max_plen, max_nlen, trace_in_lds, cols_in_lds).total;
 }

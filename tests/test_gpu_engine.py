@@ -87,3 +87,52 @@ def test_single_pair_entry_thread_switch():
             os.environ.pop("OMP_NUM_THREADS", None)
         else:
             os.environ["OMP_NUM_THREADS"] = old
+
+
+def _long_native_pair(seed, n_native_valid, n_pred_valid=300):
+    """A pair of N = n_native_valid residues, every native residue valid
+    (compact native length = n_native_valid) and n_pred_valid pred residues
+    valid, Lnorm = the same-index valid count, low noise: the pipeline's NW
+    chains run at n_pred_valid x n_native_valid, and the score stays above the
+    rescue thresholds, so the CPU oracle finishes in seconds."""
+    p = synth.make_pair(seed, n=n_native_valid, sigma=1.0)
+    rng = np.random.default_rng(seed)
+    pv = np.zeros(p.n, np.uint8)
+    start = int(rng.integers(0, p.n - n_pred_valid))
+    pv[start:start + n_pred_valid] = 1
+    p.pvalid = pv
+    p.nvalid = np.ones(p.n, np.uint8)
+    p.valid = (pv & p.nvalid).astype(np.uint8)
+    p.lnorm = float(p.valid.sum())
+    return p
+
+
+def test_native_length_limit():
+    """The NW chain kernel holds one FP64 (H, G) pair per compact native
+    residue in LDS: 10239 native-valid residues are searched (checked against
+    the oracle), 10240 give NaN and return code -6 while the other pairs of
+    the batch are searched normally."""
+    _gpu_or_skip()
+    at = _long_native_pair(60500, 10239)
+    over = _long_native_pair(60501, 10240)
+    small = synth.make_pair(60502)
+    lib = ia.load_library()
+    pk = ia._Packed([ia._as_tuple(p) for p in (small, at, over)])
+    res = (ia.NativePairResult * 3)()
+    cfg = ia.default_config()
+    rc = lib.indialign_tmscore_search_batch(3, pk.inps, ctypes.byref(cfg), res)
+    assert rc == ia.TOO_LONG
+    s, R, t = ia._results(res, 3)
+    assert np.isnan(s[2]) and np.array_equal(R[2], np.eye(3)) and not t[2].any()
+    ref = libs.oracle_search([small, at], serial_bank=1)
+    for k in range(2):
+        print(f"pair {k}: N={len(s) and [small, at][k].n} gpu {s[k]:.15f} oracle {ref[k, 12]:.15f}",
+              flush=True)
+        assert abs(s[k] - ref[k, 12]) <= 1e-6
+        assert np.abs(R[k].reshape(-1) - ref[k, :9]).max() <= 1e-5
+        assert np.abs(t[k] - ref[k, 9:12]).max() <= 1e-5
+    # the single-pair entry refuses the long pair with the same code
+    r1 = ia.NativePairResult()
+    assert lib.indialign_tmscore_search(ctypes.byref(pk.inps[2]), ctypes.byref(cfg),
+                                        ctypes.byref(r1)) == ia.TOO_LONG
+    assert np.isnan(r1.score)

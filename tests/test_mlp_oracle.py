@@ -65,3 +65,56 @@ def test_adam_textbook():
     v2 = b2 * v1[0] + (1 - b2) * 0.01
     step = lr * (m2 / (1 - b1 ** 2)) / (np.sqrt(v2 / (1 - b2 ** 2)) + eps)
     assert abs(q2[0] - (q[0] - step)) < 1e-15
+
+
+def test_forward_matches_torch_linear():
+    """The oracle's forward pass against torch.nn.functional.linear / relu on
+    the same BF16-rounded operands (torch's own conversion), accumulated in
+    float64: a transposed weight or a misplaced bias or skip term fails."""
+    import torch
+    import torch.nn.functional as F
+    rng = np.random.default_rng(21)
+    p = (rng.normal(0, 0.2, mo.NPARAM)).astype(np.float32)
+    X = rng.uniform(0, 1, (33, mo.NFEAT)).astype(np.float32)
+    W1, b1, W2, b2, w3, b3 = mo.unpack(p)
+
+    def r(a):  # BF16 operand, as float64
+        return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).to(torch.float64)
+
+    b = lambda a: torch.from_numpy(np.asarray(a, np.float64))
+    h1 = F.relu(F.linear(r(X), r(W1), b(b1))).to(torch.float32)
+    h2 = F.relu(F.linear(r(h1.numpy()), r(W2), b(b2))).to(torch.float32)
+    y = F.linear(r(h2.numpy()), r(w3.reshape(1, -1)), b(np.array([b3]))).reshape(-1)
+    y = (y + b(X[:, mo.SKIP_COL])).to(torch.float32).numpy()
+    yo, (h1o, h2o) = mo.forward(p, X)
+    np.testing.assert_allclose(h1o, h1.numpy(), rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(h2o, h2.numpy(), rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(yo, y, rtol=1e-6, atol=1e-6)
+
+
+def test_forward_hand_computed_two_features():
+    """A sparse net on two input features, values exact in BF16:
+    h1 = relu([x0, x1, x0 - x1 + 0.5]), h2 = relu([h1_0 + h1_1, h1_2]),
+    y = 2 h2_0 - h2_1 + 0.25 + x31."""
+    p = np.zeros(mo.NPARAM, np.float32)
+    W1 = p[mo.OFF_W1:mo.OFF_B1].reshape(mo.NHID, mo.NFEAT)
+    W1[0, 0] = 1.0
+    W1[1, 1] = 1.0
+    W1[2, 0], W1[2, 1] = 1.0, -1.0
+    p[mo.OFF_B1 + 2] = 0.5
+    W2 = p[mo.OFF_W2:mo.OFF_B2].reshape(mo.NHID, mo.NHID)
+    W2[0, 0] = W2[0, 1] = 1.0
+    W2[1, 2] = 1.0
+    p[mo.OFF_W3] = 2.0
+    p[mo.OFF_W3 + 1] = -1.0
+    p[mo.OFF_B3] = 0.25
+    X = np.zeros((2, mo.NFEAT), np.float32)
+    X[0, :2] = 1.5, 0.25
+    X[1, :2] = 0.25, 1.5
+    X[:, mo.SKIP_COL] = 0.125
+    y, (h1, h2) = mo.forward(p, X)
+    # row 0: h1 = [1.5, 0.25, 1.75], h2 = [1.75, 1.75], y = 3.5 - 1.75 + 0.25 + 0.125
+    # row 1: h1 = [0.25, 1.5, 0 (clipped)], h2 = [1.75, 0], y = 3.5 + 0.25 + 0.125
+    np.testing.assert_array_equal(h1[:, :3], [[1.5, 0.25, 1.75], [0.25, 1.5, 0.0]])
+    np.testing.assert_array_equal(h2[:, :2], [[1.75, 1.75], [1.75, 0.0]])
+    np.testing.assert_array_equal(y, [2.125, 3.875])
