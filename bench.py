@@ -146,6 +146,9 @@ def workload_name(kind, n):
     if kind == "short":
         return (f"{n} pairs, seeds 1..{n}, N~U{{40,300}}, sigma~U(0.5,14) A, plain FP64 method, "
                 "batch semantics")
+    if kind == "mixed":
+        return (f"{n} pairs, seeds 50001..{50000 + n}, N log-uniform over [27, 1500], "
+                "sigma~U(0.5,14) A, all masks valid, plain FP64 method, batch semantics")
     return (f"{n} pairs, seeds 20001..{20000 + n}, N~U{{400,1500}}, sigma~U(2,10) A, 20% of helix "
             "segments moved as rigid domains, 10% of pred and native residues masked, Lnorm = "
             "native valid count, plain FP64 method, batch semantics")
@@ -231,9 +234,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=None,
                     help="pairs in the batch (default 4096 short, 512 long)")
-    ap.add_argument("--workload", choices=["short", "long"], default="short",
+    ap.add_argument("--workload", choices=["short", "long", "mixed"], default="short",
                     help="short: N~U{40,300} (the headline); long: N~U{400,1500} with masks "
-                         "and domain motion")
+                         "and domain motion; mixed: N log-uniform over [27, 1500]")
     ap.add_argument("--cpu-sample", type=int, default=None,
                     help="pairs in the CPU baseline's sample (default 64 short, 16 long)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -264,9 +267,10 @@ def main():
     from indialign_amd import synth
 
     if args.pairs is None:
-        args.pairs = 4096 if args.workload == "short" else 512
+        args.pairs = {"short": 4096, "long": 512, "mixed": 1024}[args.workload]
     t0 = time.perf_counter()
-    pairs = synth.short_pairs(args.pairs) if args.workload == "short" else synth.long_pairs(args.pairs)
+    pairs = {"short": synth.short_pairs, "long": synth.long_pairs,
+             "mixed": synth.mixed_pairs}[args.workload](args.pairs)
     log(f"[bench] generated {len(pairs)} pairs in {time.perf_counter() - t0:.1f} s")
     if not ia.gpu_available():
         sys.exit("bench.py: no GPU visible")
@@ -342,7 +346,7 @@ def main():
     if not args.no_single:
         out["single_pair"] = single_pair_latency(ia, pairs)
     if not args.no_cpu:
-        sample = args.cpu_sample or (64 if args.workload == "short" else 16)
+        sample = args.cpu_sample or {"short": 64, "long": 16, "mixed": 32}[args.workload]
         log(f"[bench] cpu baseline on {sample} pairs ...")
         out["cpu_baseline"] = cpu_baseline(pairs, sample)
     print(json.dumps(out), flush=True)

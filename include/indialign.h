@@ -251,6 +251,14 @@ int indialign_set_learned_options(int reduce_rescored, int fallback_top_m,
  * range (the setting is then unchanged). */
 int indialign_set_launch_tuning(int chain_waves, int chain_slots_per_cu, int chain_trace,
                                 int bank_pass_cap);
+/* Length classes of the NW chain launches (default 1, also read from the
+ * tuning file as "chain_buckets N"): chains are split by their pair's
+ * compact native length (<= 160, <= 400, <= 900, longer; n_classes merges
+ * the longest classes) into concurrent launches, each with the LDS plan and
+ * occupancy of its own longest chain; 1 = one launch for every chain.
+ * Results do not depend on it.  Returns 0, or -1 for n_classes outside 1..4. */
+int indialign_set_chain_buckets(int n_classes);
+
 /* Current launch shape into out4[0..3] (file or defaults 3, 0, 0, 5). */
 int indialign_get_launch_tuning(int *out4);
 

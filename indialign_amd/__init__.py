@@ -102,6 +102,8 @@ def load_library(path: str | None = None):
     lib.indialign_set_launch_tuning.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int]
     lib.indialign_set_launch_tuning.restype = ctypes.c_int
+    lib.indialign_set_chain_buckets.argtypes = [ctypes.c_int]
+    lib.indialign_set_chain_buckets.restype = ctypes.c_int
     lib.indialign_get_launch_tuning.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.indialign_get_launch_tuning.restype = ctypes.c_int
     lib.indialign_collect_records.argtypes = [
@@ -338,6 +340,14 @@ def set_launch_tuning(chain_waves: int = 3, chain_slots_per_cu: int = 0,
     if rc != 0:
         raise IndialignError(f"indialign_set_launch_tuning({chain_waves}, {chain_slots_per_cu}, "
                              f"{chain_trace}, {bank_pass_cap}) returned {rc}")
+
+
+def set_chain_buckets(n_classes: int = 1) -> None:
+    """Length classes of the NW chain launches (indialign_set_chain_buckets);
+    results do not depend on it."""
+    rc = load_library().indialign_set_chain_buckets(int(n_classes))
+    if rc != 0:
+        raise IndialignError(f"indialign_set_chain_buckets({n_classes}) returned {rc}")
 
 
 def get_launch_tuning() -> tuple[int, int, int, int]:

@@ -96,7 +96,8 @@ constexpr double kNever = 1e300;  // early termination disabled
 }  // namespace
 
 constexpr int kNStats = 11;
-constexpr int kTooLong = -6;  // return code: some pair exceeds chain_max_nlen()
+constexpr int kTooLong = -6;      // return code: some pair exceeds chain_max_nlen()
+constexpr int kChainClasses = 4;  // compact native length <= 160, 400, 900, longer
 
 class Engine {
   public:
@@ -107,11 +108,19 @@ class Engine {
         IA_CHECK(hipDeviceGetAttribute(&n_cu_, hipDeviceAttributeMultiprocessorCount, dev));
         IA_CHECK(hipEventCreate(&e0_));
         IA_CHECK(hipEventCreate(&e1_));
+        for (int k = 0; k < kChainClasses; k++) {
+            IA_CHECK(hipStreamCreateWithFlags(&cs_[k], hipStreamNonBlocking));
+            IA_CHECK(hipEventCreateWithFlags(&cev_[k], hipEventDisableTiming));
+        }
     }
     ~Engine() {
         for (DBuf *b : all_bufs()) b->release();
         (void)hipEventDestroy(e0_);
         (void)hipEventDestroy(e1_);
+        for (int k = 0; k < kChainClasses; k++) {
+            (void)hipEventDestroy(cev_[k]);
+            (void)hipStreamDestroy(cs_[k]);
+        }
         (void)hipStreamDestroy(s_);
     }
 
@@ -160,6 +169,11 @@ class Engine {
     int chain_waves = 3;
     int chain_slots_per_cu = 0;
     int chain_trace = 0;
+    // length classes of the chain launches (1: one launch for every chain,
+    // the default: one queue ordered longest first balances the short-pair
+    // batch best; the classes only pay off when a few very long chains
+    // would set a low-occupancy LDS plan for everything)
+    int chain_buckets = 1;
     // Seed banks: refinement passes a candidate runs in the first launch
     // before it is left to the continuation launch (0: one launch).
     int bank_pass_cap = 5;
@@ -177,6 +191,9 @@ class Engine {
   private:
     hipStream_t s_;
     hipEvent_t e0_, e1_;
+    // streams of the concurrent chain launches of the length classes
+    hipStream_t cs_[kChainClasses] = {};
+    hipEvent_t cev_[kChainClasses] = {};
     int n_cu_ = 256;
     const NativeConfig *cfg_ = nullptr;
     int serial_ = 1;
@@ -612,33 +629,59 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
 }
 
 long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev, Hit *out_dev) {
-    // The queue hands out chains longest first (NW cells x iterations per
-    // chain, stable among equals), so a launch ends on its cheapest chains
-    // instead of waiting on a long one that came last.  Each chain writes
-    // only its own outputs, so the order changes no result; chain_off is
-    // the prefix in queue order.
-    std::vector<int> order(groups.size());
-    for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
+    // Length classes: chains are split by their pair's compact native length
+    // into up to chain_buckets launches, each with the LDS plan (trace in
+    // LDS or a global slot, staged columns) and occupancy of its own longest
+    // chain, so one long pair no longer sets the plan of every short chain.
+    // The launches run concurrently on their own streams (default priority);
+    // each persistent grid pulls from its own queue.  Within a class the
+    // queue hands out chains longest first (NW cells x iterations per
+    // chain, stable among equals), so a launch ends on its cheapest chains.
+    // Each chain writes only its own outputs, so neither the classes nor the
+    // order change any result; chain_off is the prefix in queue order.
+    static const int kBound[kChainClasses] = {160, 400, 900, 1 << 30};
+    const int ncls = std::max(1, std::min(chain_buckets, kChainClasses));
+    auto cls_of = [&](const ChainGroup &g) {
+        if (ncls == 1) return 0;
+        const int n = hp_[g.pair].nlen;
+        int c = 0;
+        while (c < kChainClasses - 1 && n > kBound[c]) c++;
+        return std::min(c, ncls - 1);
+    };
     auto chain_cost = [&](const ChainGroup &g) {
         const HostPair &h = hp_[g.pair];
         return (double)h.plen * h.nlen * g.iters;
     };
+    std::vector<int> order(groups.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const int ca = cls_of(groups[a]), cb = cls_of(groups[b]);
+        if (ca != cb) return ca < cb;
         return chain_cost(groups[a]) > chain_cost(groups[b]);
     });
+    struct Cls {
+        long long g0 = 0, ng = 0, total = 0;
+        int mp = 1, mn = 1;
+        bool in_lds = false, cols_lds = false;
+        long long slots = 0, slot_words = 0, trace_off = 0;
+        size_t lds = 0;
+    };
+    std::vector<Cls> cl(kChainClasses);
     std::vector<ChainGroup> queued;
     queued.reserve(groups.size());
-    long long total = 0;
-    int mp = 1, mn = 1, lc = 1;
-    for (int k : order) {
-        ChainGroup &g = groups[k];
-        g.chain_off = total;
+    long long total_all = 0;
+    for (size_t q = 0; q < order.size(); q++) {
+        ChainGroup &g = groups[order[q]];
+        Cls &c = cl[cls_of(g)];
+        if (c.ng == 0) c.g0 = (long long)q;
+        c.ng++;
+        g.chain_off = c.total;
         queued.push_back(g);
-        total += g.n_chains;
+        c.total += g.n_chains;
+        total_all += g.n_chains;
         const HostPair &h = hp_[g.pair];
-        mp = std::max(mp, h.plen);
-        mn = std::max(mn, h.nlen);
-        lc = std::max(lc, std::min(h.plen, h.nlen));
+        c.mp = std::max(c.mp, h.plen);
+        c.mn = std::max(c.mn, h.nlen);
         long long filled = g.n_chains;  // chains that run the NW fill
         if (g.init == CI_OFFSET && g.k0 <= 0 && (-g.k0) % g.kstep == 0 &&
             (-g.k0) / g.kstep < g.n_chains / 2)
@@ -646,56 +689,79 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         stats[3] += g.n_chains;
         stats[2] += (double)h.plen * h.nlen * (double)filled * g.iters;
     }
-    if (total == 0) return 0;
-    const ChainGroup *dg = upload(b_groups, queued);
-    int *q = b_queue.get<int>(1);
-    IA_CHECK(hipMemsetAsync(q, 0, sizeof(int), s_));
-    const long long trace_w = chain_trace_words(mp, mn);
-    const size_t trace_bytes = (size_t)trace_w * 4;
-    // LDS layout: the trace and the staged columns each go to LDS only when
-    // that costs no resident wavefront (the fill is latency-bound, so
-    // occupancy beats the global slots' extra traffic); otherwise the trace
-    // lives in a per-wavefront global slot and the columns are read from the
-    // arena.
+    if (total_all == 0) return 0;
+    // LDS layout per class: the trace and the staged columns each go to LDS
+    // only when that costs no resident wavefront (the fill is latency-bound,
+    // so occupancy beats the global slots' extra traffic); otherwise the
+    // trace lives in a per-wavefront global slot and the columns are read
+    // from the arena.
     constexpr size_t kLdsMax = 160 * 1024;
-    auto occupancy = [&](bool tr, bool cl) {
-        const size_t b = chain_lds_bytes(mp, mn, tr, cl);
-        return b > kLdsMax ? 0 : chain_blocks_per_cu(b, fast_, chain_waves);
-    };
-    const int occ_free = occupancy(false, false);
-    if (occ_free <= 0) throw DevError("chain too long for one workgroup's LDS");
-    const bool lds_fits = trace_bytes <= 96 * 1024 && occupancy(true, false) > 0;
-    const bool in_lds = chain_trace == 2 ? false
-                        : chain_trace == 1 ? lds_fits
-                                           : lds_fits && occupancy(true, false) >= occ_free;
-    const bool cols_lds = occupancy(in_lds, true) >= occupancy(in_lds, false);
-    const size_t lds = chain_lds_bytes(mp, mn, in_lds, cols_lds);
-    int per_cu = std::max(1, chain_blocks_per_cu(lds, fast_, chain_waves));
-    per_cu = std::min(per_cu, 16);
-    if (chain_slots_per_cu > 0) per_cu = std::min(per_cu, chain_slots_per_cu);
-    long long slots = (long long)n_cu_ * per_cu;
-    unsigned *trace = nullptr;
-    long long slot_words = 0;
-    if (!in_lds) {
-        slot_words = trace_w;
-        const long long budget = 8LL << 30;  // bytes of global trace slots
-        slots = std::max(1LL, std::min(slots, budget / (slot_words * 4)));
-        trace = b_trace.get<unsigned>((size_t)(slots * slot_words));
+    int active = 0;
+    for (const Cls &c : cl) active += c.total > 0 ? 1 : 0;
+    const long long budget = (8LL << 30) / std::max(1, active);  // bytes of global trace slots
+    long long trace_words_all = 0;
+    for (Cls &c : cl) {
+        if (c.total == 0) continue;
+        auto occupancy = [&](bool tr, bool co) {
+            const size_t b = chain_lds_bytes(c.mp, c.mn, tr, co);
+            return b > kLdsMax ? 0 : chain_blocks_per_cu(b, fast_, chain_waves);
+        };
+        const int occ_free = occupancy(false, false);
+        if (occ_free <= 0) throw DevError("chain too long for one workgroup's LDS");
+        const long long trace_w = chain_trace_words(c.mp, c.mn);
+        const bool lds_fits = trace_w * 4 <= 96 * 1024 && occupancy(true, false) > 0;
+        c.in_lds = chain_trace == 2 ? false
+                   : chain_trace == 1 ? lds_fits
+                                      : lds_fits && occupancy(true, false) >= occ_free;
+        c.cols_lds = occupancy(c.in_lds, true) >= occupancy(c.in_lds, false);
+        c.lds = chain_lds_bytes(c.mp, c.mn, c.in_lds, c.cols_lds);
+        int per_cu = std::max(1, chain_blocks_per_cu(c.lds, fast_, chain_waves));
+        per_cu = std::min(per_cu, 16);
+        if (chain_slots_per_cu > 0) per_cu = std::min(per_cu, chain_slots_per_cu);
+        c.slots = (long long)n_cu_ * per_cu;
+        if (!c.in_lds) {
+            c.slot_words = trace_w;
+            c.slots = std::max(1LL, std::min(c.slots, budget / (c.slot_words * 4)));
+        }
+        c.slots = std::min<long long>(c.slots, c.total);
+        c.trace_off = trace_words_all;
+        if (!c.in_lds) trace_words_all += c.slots * c.slot_words;
     }
-    slots = std::min<long long>(slots, total);
+    // every buffer is sized before the first launch: the launches of the
+    // classes are in flight together
+    const ChainGroup *dg = upload(b_groups, queued);
+    int *q = b_queue.get<int>(kChainClasses);
+    unsigned *trace = trace_words_all ? b_trace.get<unsigned>((size_t)trace_words_all) : nullptr;
+    ListOut *lout = b_lout.get<ListOut>(1);
+    int *lp = b_lists_p.get<int>(1), *ln = b_lists_n.get<int>(1), *lna = b_list_na.get<int>(1);
+    IA_CHECK(hipMemsetAsync(q, 0, sizeof(int) * kChainClasses, s_));
     IA_CHECK(hipEventRecord(e0_, s_));
-    launch_chains(s_, A_, dg, (int)groups.size(), total, xf_dev, out_dev,
-                  b_lout.get<ListOut>(1), b_lists_p.get<int>(1), b_lists_n.get<int>(1),
-                  b_list_na.get<int>(1), list_cap_, mp, mn, q, trace, slot_words, (int)slots,
-                  cols_lds, fast_, chain_waves);
-    launched();
-    IA_CHECK(hipGetLastError());
+    int k_stream = 0;
+    for (int ci = 0; ci < kChainClasses; ci++) {
+        const Cls &c = cl[ci];
+        if (c.total == 0) continue;
+        hipStream_t st = s_;
+        if (active > 1) {
+            st = cs_[k_stream];
+            IA_CHECK(hipStreamWaitEvent(st, e0_, 0));
+        }
+        launch_chains(st, A_, dg + c.g0, (int)c.ng, c.total, xf_dev, out_dev, lout, lp, ln, lna,
+                      list_cap_, c.mp, c.mn, q + ci, c.in_lds ? nullptr : trace + c.trace_off,
+                      c.slot_words, (int)c.slots, c.cols_lds, fast_, chain_waves);
+        launched();
+        IA_CHECK(hipGetLastError());
+        if (active > 1) {
+            IA_CHECK(hipEventRecord(cev_[k_stream], st));
+            IA_CHECK(hipStreamWaitEvent(s_, cev_[k_stream], 0));
+            k_stream++;
+        }
+    }
     IA_CHECK(hipEventRecord(e1_, s_));
     IA_CHECK(hipEventSynchronize(e1_));
     float ms = 0;
     IA_CHECK(hipEventElapsedTime(&ms, e0_, e1_));
     stats[5] += ms;
-    return total;
+    return total_all;
 }
 
 std::vector<Hit> Engine::rescore(const std::vector<int> &pairs, const std::vector<Hit> &in) {
@@ -1626,6 +1692,7 @@ unsigned g_engine_ranker_ver = 0;
 // placement, seed-bank pass cap} (Engine::chain_*, Engine::bank_pass_cap).  Read once from a tuning file written by the autotuner,
 // then settable with indialign_set_launch_tuning.
 int g_tune[4] = {3, 0, 0, 5};
+int g_chain_buckets = 1;
 bool g_tune_loaded = false;
 
 bool tune_valid(int waves, int slots, int trace, int cap) {
@@ -1666,6 +1733,8 @@ void load_tuning_file() {
         else if (!std::strcmp(key, "chain_slots_per_cu")) t[1] = v;
         else if (!std::strcmp(key, "chain_trace")) t[2] = v;
         else if (!std::strcmp(key, "bank_pass_cap")) t[3] = v;
+        else if (!std::strcmp(key, "chain_buckets") && v >= 1 && v <= ia::kChainClasses)
+            g_chain_buckets = v;
     }
     std::fclose(f);
     if (ok && tune_valid(t[0], t[1], t[2], t[3])) {
@@ -1712,6 +1781,7 @@ void sync_engine() {
     g_engine->chain_slots_per_cu = g_tune[1];
     g_engine->chain_trace = g_tune[2];
     g_engine->bank_pass_cap = g_tune[3];
+    g_engine->chain_buckets = g_chain_buckets;
     if (g_engine_synced && g_engine_ranker_ver == g_ranker_ver) return;
     if (g_ranker.empty()) g_engine->ranker.unload();
     else g_engine->ranker.set_params(g_engine->stream(), g_ranker.data());
@@ -1998,6 +2068,14 @@ int indialign_set_launch_tuning(int chain_waves, int chain_slots_per_cu, int cha
     g_tune[1] = chain_slots_per_cu;
     g_tune[2] = chain_trace;
     g_tune[3] = bank_pass_cap;
+    return 0;
+}
+
+int indialign_set_chain_buckets(int n_classes) {
+    if (n_classes < 1 || n_classes > ia::kChainClasses) return -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    load_tuning_file();
+    g_chain_buckets = n_classes;
     return 0;
 }
 

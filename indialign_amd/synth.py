@@ -11,6 +11,8 @@ workload is the same on every machine and independent of batch size.
 Workloads (sizes as the bench and the parity tests use them):
   single   1 pair, seed 0, N=64, sigma=3 A, all masks valid, Lnorm=N
   short    pairs with seeds 1..n, N~U{40,300}, sigma~U(0.5,14) A
+  mixed    pairs with seeds 50001.., N log-uniform over [27, 1500],
+           sigma~U(0.5,14) A, all masks valid
   long     pairs with seeds 20001.., N~U{400,1500}, sigma~U(2,10) A, 20% of
            helix segments moved rigidly (20-60 deg about a random axis through
            the segment centroid, 5-15 A shift), 10% of pred-valid and of
@@ -160,6 +162,17 @@ def single_pair() -> Pair:
 
 def short_pairs(count: int, first_seed: int = 1) -> list[Pair]:
     return [make_pair(s) for s in range(first_seed, first_seed + count)]
+
+
+def mixed_pairs(count: int, first_seed: int = 50001) -> list[Pair]:
+    """Pairs of uneven length: N log-uniform over [27, 1500] (as many pairs
+    of 27-100 nt as of 400-1500 nt), sigma~U(0.5,14) A, all masks valid."""
+    out = []
+    for s in range(first_seed, first_seed + count):
+        u = float(_rng(s + 7_000_000).uniform())
+        n = int(round(np.exp(np.log(27.0) + u * (np.log(1500.0) - np.log(27.0)))))
+        out.append(make_pair(s, n=n))
+    return out
 
 
 def long_pairs(count: int, first_seed: int = 20001) -> list[Pair]:

@@ -73,20 +73,24 @@ def test_candidate_shapes():
 
 @pytest.mark.gpu
 def test_launch_shapes_give_identical_results():
-    """Short pairs plus one long masked pair, plain and fast methods: every
-    launch shape (chain kernel variant, slots, trace placement, seed-bank pass
-    cap with its compacted continuation launch) returns the same scores,
-    rotations and translations."""
+    """Short pairs, one long masked pair and mixed-length pairs, plain and
+    fast methods: every launch shape (chain kernel variant, slots, trace
+    placement, seed-bank pass cap with its compacted continuation launch,
+    chain length classes) returns the same scores, rotations and
+    translations."""
     if not ia.gpu_available():
         pytest.fail("no GPU visible to the HIP runtime")
     before = ia.get_launch_tuning()
-    pairs = synth.short_pairs(24) + synth.long_pairs(1)
+    # lengths in every chain length class (<= 160, <= 400, <= 900, longer)
+    pairs = synth.short_pairs(24) + synth.long_pairs(1) + synth.mixed_pairs(6, first_seed=50101)
     try:
         for method in (0, 1):
             ref = None
             for shape in [(3, 0, 0, 0), (2, 0, 0, 0), (3, 4, 2, 0), (2, 8, 1, 2), (3, 0, 1, 3),
-                          (3, 0, 0, 1), (3, 0, 0, 6)]:
-                ia.set_launch_tuning(*shape)
+                          (3, 0, 0, 1), (3, 0, 0, 6), (3, 0, 0, 5, 1), (3, 0, 0, 5, 2),
+                          (3, 0, 2, 5, 4)]:
+                ia.set_launch_tuning(*shape[:4])
+                ia.set_chain_buckets(shape[4] if len(shape) > 4 else 1)
                 out = ia.tmscore_search_batch(pairs, method=method)
                 print(f"method {method} shape {shape}: mean score {out[0].mean():.12f}", flush=True)
                 if ref is None:
@@ -96,3 +100,4 @@ def test_launch_shapes_give_identical_results():
                     assert np.array_equal(a, b), (method, shape)
     finally:
         ia.set_launch_tuning(*before)
+        ia.set_chain_buckets(1)
