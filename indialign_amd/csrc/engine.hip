@@ -97,6 +97,7 @@ constexpr double kNever = 1e300;  // early termination disabled
 
 constexpr int kNStats = 11;
 constexpr int kTooLong = -6;      // return code: some pair exceeds chain_max_nlen()
+constexpr int kUnsupported = -7;  // return code: method 3 with dp_iter > 0 or separate masks
 constexpr int kChainClasses = 4;  // compact native length <= 160, 400, 900, longer
 
 class Engine {
@@ -200,6 +201,30 @@ class Engine {
     int omp_threads_ = 1;
     bool fast_ = false;
     bool learned_ = false;
+    // method 3: the seed search of the reference's PyTorch engine
+    // (scoring/_tmscore_gpu/search.py:492-761 with dp_iter = 0 and
+    // same-index masks): window seeds, no anchor seeds, cutoffs d0_search x
+    // {2, 1.5, 1, .75, .5, .25} only, a fixed number of refinement passes
+    // (no convergence break, search.py:423-489), candidates ranked by the
+    // TM-score with the score_d8 cut (runtime.py:40), then the top-k seed,
+    // and the reported score max(TM-score without the cut, ranking score)
+    // (search.py:744-757).
+    bool torch_sem_ = false;
+    void bank_cutoffs(BankDesc &b, const HostPair &h) const {
+        static const double mult[6] = {2.0, 1.5, 1.0, 0.75, 0.5, 0.25};
+        if (torch_sem_) {
+            b.C = 6;
+            for (int c = 0; c < 6; c++) b.cut[c] = h.info.d0s * mult[c];
+            b.use_d8 = 1;
+            b.fixed_iters = 1;
+        } else {
+            b.C = h.info.C;
+            std::memcpy(b.cut, h.info.cut, sizeof(b.cut));
+            b.use_d8 = 0;
+            b.fixed_iters = 0;
+        }
+        b.sd8 = h.info.sd8;
+    }
     std::vector<HostPair> hp_;
     std::vector<Hit> best_;
     std::vector<int> live_;  // pairs still in the pipeline (index into hp_)
@@ -968,10 +993,14 @@ void Engine::stage_seed_bank() {
     const long long *dso = upload(b_soff, soff);
     uint8_t *masks = b_masks.get<uint8_t>(abytes);
     int *dac = b_acnt.get<int>(n);
-    launch_anchors(s_, A_, dpairs, n, dfrag, dfo, dfc, masks, dao, dac, b_ascr.get<int>(sints), dso, max_n);
-    launched();
-    IA_CHECK(hipGetLastError());
-    const std::vector<int> acnt = download(dac, n);
+    std::vector<int> acnt(n, 0);
+    if (!torch_sem_) {  // the PyTorch engine builds no anchor seeds by default
+        launch_anchors(s_, A_, dpairs, n, dfrag, dfo, dfc, masks, dao, dac, b_ascr.get<int>(sints),
+                       dso, max_n);
+        launched();
+        IA_CHECK(hipGetLastError());
+        acnt = download(dac, n);
+    }
     std::vector<SeedDesc> seeds;
     std::vector<BankDesc> banks;
     for (int a = 0; a < n; a++) {
@@ -985,17 +1014,14 @@ void Engine::stage_seed_bank() {
         b.seeds = SEED_EXPLICIT;
         b.seed_off = (int)seeds.size();
         b.K = K;
-        b.C = h.info.C;
+        bank_cutoffs(b, h);
         b.iters = cfg_->max_iter;
-        b.use_d8 = 0;
         b.init = INIT_SEED;
-        b.cand_cap = K * h.info.C;
+        b.cand_cap = K * b.C;
         b.out_idx = a;
         const bool serial = serial_ >= 0 ? serial_ != 0
                                           : !(omp_threads_ > 1 && K * h.info.C >= 4 * omp_threads_);
-        b.et = serial ? cfg_->early_term_score : kNever;
-        b.sd8 = h.info.sd8;
-        std::memcpy(b.cut, h.info.cut, sizeof(b.cut));
+        b.et = serial && !torch_sem_ ? cfg_->early_term_score : kNever;
         banks.push_back(b);
         seeds.insert(seeds.end(), h.frag.begin(), h.frag.end());
         for (int r = 0; r < acnt[a]; r++) seeds.push_back(SeedDesc{(int)(aoff[a] + (long long)r * h.N), -1});
@@ -1469,15 +1495,12 @@ void Engine::stage_topk() {
             b.seed_off = (int)seeds.size();
             seeds.push_back(SeedDesc{(int)moff[k], -1});
             b.K = 1;
-            b.C = h.info.C;
-            std::memcpy(b.cut, h.info.cut, sizeof(b.cut));
+            bank_cutoffs(b, h);
             b.iters = cfg_->max_iter;
-            b.use_d8 = 0;
             b.init = INIT_SEED;
-            b.cand_cap = h.info.C;
+            b.cand_cap = b.C;
             b.out_idx = (long long)k;
-            b.et = cfg_->early_term_score;
-            b.sd8 = h.info.sd8;
+            b.et = torch_sem_ ? kNever : cfg_->early_term_score;
             banks.push_back(b);
         }
         Hit *d = run_banks(banks, seeds, P.size());
@@ -1630,6 +1653,12 @@ int Engine::search(const NativeConfig *cfg, int serial_bank, NativePairResult *o
     omp_threads_ = omp_threads;
     fast_ = method == 1 || method == 2;
     learned_ = method == 2;
+    torch_sem_ = method == 3;
+    if (torch_sem_) {  // only the engine's seed search (dp_iter = 0, same-index masks) is rebuilt
+        if (cfg->dp_iter > 0) return kUnsupported;
+        for (const HostPair &h : hp_)
+            if (h.cross) return kUnsupported;
+    }
     if (learned_ && !ranker.loaded())
         throw DevError("learned method requested but no ranker is loaded (indialign_set_ranker)");
     stage_scores.assign((size_t)n_loaded_ * kStages, std::nan(""));
@@ -1649,12 +1678,26 @@ int Engine::search(const NativeConfig *cfg, int serial_bank, NativePairResult *o
         IA_CHECK(hipMemsetAsync(A_.bank_work, 0, 2 * sizeof(unsigned long long), s_));
         enumerate_seeds();
         stage_seed_bank();
+        if (torch_sem_) {
+            stage_topk();
+            // reported score: the best of the TM-score without the d8 cut and
+            // the ranking score (search.py:754-757)
+            const std::vector<Hit> rs = rescore(live_, [&] {
+                std::vector<Hit> v;
+                for (int a : live_) v.push_back(best_[a]);
+                return v;
+            }());
+            for (size_t k = 0; k < live_.size(); k++)
+                best_[live_[k]].score = std::max(best_[live_[k]].score, rs[k].score);
+            goto done;
+        }
         stage_ss();
         stage_gapless();
         stage_rescue();
         stage_topk();
         stage_dp();
         stage_weighted();
+    done:
         const std::vector<unsigned long long> bw = download(A_.bank_work, 2);
         stats[9] = (double)bw[1];
         stats[10] = (double)bw[0];
@@ -1887,7 +1930,7 @@ int indialign_tmscore_search_batch(int n_pairs, const NativePairInput *inputs,
 int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
                         NativePairResult *outputs, int serial_bank, int method) {
     if (n_pairs <= 0) return 0;
-    if (method < 0 || method > 2) return -3;
+    if (method < 0 || method > 3) return -3;
     return run_search(n_pairs, inputs, cfg, outputs, serial_bank != 0 ? 1 : 0, method);
 }
 
@@ -1911,7 +1954,7 @@ int indialign_batch_load(int n_pairs, const NativePairInput *inputs) {
 int indialign_batch_run(int n_pairs, const NativeConfig *cfg, NativePairResult *outputs,
                         int serial_bank, int method) {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (method < 0 || method > 2) return -3;
+    if (method < 0 || method > 3) return -3;
     if (method == 2 && g_ranker.empty()) return -4;
     if (!g_engine || !g_resident || n_pairs != g_resident_n) return -5;
     try {

@@ -26,6 +26,7 @@ struct BankDesc {
     int seeds, seed_off, K, max_starts;
     int C, iters, use_d8, init, min_ali;
     int cand_cap;
+    int fixed_iters;     // 1: no convergence break, every candidate runs `iters` passes
     long long cand_off;  // first candidate slot
     long long out_idx;   // slot of the reduced result
     double et;           // early termination threshold (search.h:83); huge = never

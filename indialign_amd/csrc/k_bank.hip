@@ -142,7 +142,7 @@ IA_D bool refine_passes(const Arena &A, const BankDesc &B, int n, int c, const P
             }
         }
         if (cnt >= 3) kops += 31 * cnt + 3400;  // centroid + covariance + solve
-        if (cnt == last) {
+        if (cnt == last && !B.fixed_iters) {
             it++;
             return true;
         }

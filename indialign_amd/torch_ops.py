@@ -7,11 +7,20 @@ d0 / d0_search / score_d8 / Lnorm tensors, the refinement and DP knobs, and
 optional separate pred/native masks.  It returns the (B,) best TM-scores on
 pred's device and dtype, like the reference function.
 
-The search that runs is this library's C-ABI search (the reference's
-indialign_tmscore_search semantics, method 0 by default, FP64), not the
-torch engine's own algorithm: the two differ in stage list, seed schedule and
-refinement details, so scores differ from the torch engine's on the same
-inputs.  The C search derives d0, d0_search and score_d8 from Lnorm with the
+Two semantics are selectable.  semantics="c" (default) runs this library's
+C-ABI search (the reference's indialign_tmscore_search semantics, method 0
+by default, FP64): its stage list, seed schedule and refinement differ from
+the torch engine's, so scores differ from that engine's on the same inputs.
+semantics="torch" runs the torch engine's own seed search
+(scoring/_tmscore_gpu/search.py:492-761 with dp_iter = 0; library method 3):
+window seeds without anchor seeds, cutoffs d0_search x {2, 1.5, 1, .75, .5,
+.25} only, exactly max_iter refinement passes per candidate with no
+convergence break, candidates ranked by the TM-score with the score_d8 cut
+(the engine's _SEARCH_SELECTION_USES_SCORE_D8 default), the top-k distance
+seed, and the reported score max(TM-score without the cut, ranking score).
+The engine's DP refinement and cross-index rescue passes are not rebuilt:
+semantics="torch" with dp_iter > 0 or separate masks raises
+NotImplementedError.  The C search derives d0, d0_search and score_d8 from Lnorm with the
 closed forms of core.h:12-25, which are also the reference's
 d0_from_length (scoring/_tmscore_gpu/runtime.py:91-134); the given tensors
 are checked against them and a different rule raises ValueError, since the
@@ -28,7 +37,7 @@ import math
 
 import numpy as np
 
-from . import METHOD_PLAIN, default_config, tmscore_search_batch
+from . import METHOD_PLAIN, METHOD_TORCH_SEED, default_config, tmscore_search_batch
 
 # Python reference MAX_ITER default (scoring/_tmscore_gpu/runtime.py: _MAX_ITER)
 DEFAULT_MAX_ITER = 20
@@ -62,7 +71,7 @@ def _host(x, dtype):
 def tmscore_search(pred, native, valid, d0, d0_search, score_d8, Lnorm, max_iter=None,
                    use_fragment_search: bool = True, max_mem_gb: float = 20.0, dp_iter: int = 0,
                    pred_valid=None, native_valid=None, method: int = METHOD_PLAIN,
-                   rtol: float = 1e-5):
+                   rtol: float = 1e-5, semantics: str = "c"):
     """Best TM-score per pair of a padded batch (see the module docstring).
 
     pred, native: (B, N, 3); valid: (B, N) bool, pred/native same-index
@@ -72,8 +81,22 @@ def tmscore_search(pred, native, valid, d0, d0_search, score_d8, Lnorm, max_iter
     max_mem_gb is accepted and unused (device memory is sized by the batch).
     method: 0 plain FP64 (default), 1 fast, 2 learned (needs a ranker).
     rtol: relative tolerance of the d0 / d0_search / score_d8 check (the
-    reference computes them in float32 by default)."""
+    reference computes them in float32 by default).
+    semantics: "c" (the C path, `method` applies) or "torch" (the torch
+    engine's seed search, FP64; dp_iter must be 0 and no separate masks)."""
     import torch
+
+    if semantics not in ("c", "torch"):
+        raise ValueError('semantics must be "c" or "torch"')
+    if semantics == "torch":
+        if int(dp_iter) != 0:
+            raise NotImplementedError("semantics='torch': the torch engine's DP refinement "
+                                      "(dp_iter > 0) is not rebuilt")
+        for m in (pred_valid, native_valid):
+            if m is not None and not bool(torch.equal(m.bool(), valid.bool())):
+                raise NotImplementedError("semantics='torch': the torch engine's cross-index "
+                                          "rescue (separate masks) is not rebuilt")
+        method = METHOD_TORCH_SEED
 
     if max_iter is None:
         max_iter = DEFAULT_MAX_ITER

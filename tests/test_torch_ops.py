@@ -63,6 +63,45 @@ def test_argument_checks_without_device():
     # empty batch
     out = torch_ops.tmscore_search(pred[:0], native[:0], valid[:0], d0[:0], d0s[:0], d8[:0], L[:0])
     assert out.shape == (0,) and out.dtype == pred.dtype
+    # the torch engine's semantics: its DP refinement and cross-index rescue
+    # are not rebuilt, and an unknown semantics is refused
+    with pytest.raises(NotImplementedError):
+        torch_ops.tmscore_search(pred, native, valid, d0, d0s, d8, L, dp_iter=1, semantics="torch")
+    pv2 = pv.clone()
+    pv2[0, 0] = ~pv2[0, 0]
+    with pytest.raises(NotImplementedError):
+        torch_ops.tmscore_search(pred, native, valid, d0, d0s, d8, L, pred_valid=pv2,
+                                 native_valid=nv, semantics="torch")
+    with pytest.raises(ValueError):
+        torch_ops.tmscore_search(pred, native, valid, d0, d0s, d8, L, semantics="numpy")
+
+
+TOY_PRED = [[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [1.0, 1.0, 0.0], [0.5, 0.2, 0.3]]
+TOY_NATIVE = [[2.0, 3.0, 4.0], [2.8, 3.4, 4.1], [1.7, 4.1, 4.2], [2.5, 4.5, 4.3], [2.1, 3.6, 4.6]]
+
+
+@pytest.mark.gpu
+def test_torch_semantics_reference_snapshot():
+    """The reference's own snapshot of its torch engine
+    (tests/test_gpu_scoring_refactor.py:176-208: the 5-residue toy, float64,
+    max_iter 20, fragment search, dp_iter 0) scores 0.7666330261384652;
+    semantics="torch" reproduces it within the reference test's 1e-12, and
+    the default C semantics give the C path's 0.77317584653223026 on the
+    same inputs."""
+    if not ia.gpu_available():
+        pytest.fail("no GPU visible to the HIP runtime (GPU tests must run on the MI355X)")
+    pred = torch.tensor([TOY_PRED], dtype=torch.float64)
+    native = torch.tensor([TOY_NATIVE], dtype=torch.float64)
+    valid = torch.ones((1, 5), dtype=torch.bool)
+    L = valid.sum(dim=1).to(torch.float64)
+    d = torch.tensor([torch_ops.d0_params(5.0)], dtype=torch.float64)
+    args = (pred, native, valid, d[:, 0], d[:, 1], d[:, 2], L)
+    kw = dict(max_iter=20, use_fragment_search=True, max_mem_gb=4.0, dp_iter=0)
+    st = torch_ops.tmscore_search(*args, semantics="torch", **kw)
+    sc = torch_ops.tmscore_search(*args, **kw)
+    print(f"torch semantics {float(st[0])!r}, C semantics {float(sc[0])!r}", flush=True)
+    assert abs(float(st[0]) - 0.7666330261384652) <= 1e-12
+    assert abs(float(sc[0]) - 0.77317584653223026) <= 1e-12
 
 
 @pytest.mark.gpu
