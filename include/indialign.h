@@ -59,8 +59,9 @@ void indialign_default_config(NativeConfig *cfg);
 
 /* Replaces indialign_tmscore_search (api.h:39-41, indialign.cpp:284-303).
  * Returns -1 when length <= 0 (as the reference), -2 when no usable GPU is
- * present or a device error occurred (message on stderr; the reference
- * never had this case because it fell back to its CPU path), else 0.
+ * present or a device error occurred (message on stderr; out then holds
+ * score NaN and the identity; the reference never had this case because it
+ * fell back to its CPU path), else 0.
  * The main seed bank follows the reference's per-call switch
  * (search.h:14-24, 83-136): early termination at early_term_score (its
  * serial path) when the reference's OpenMP runtime would run one thread or
@@ -80,8 +81,10 @@ int indialign_tmscore_search(const NativePairInput *inp, const NativeConfig *cfg
  * indialign.cpp:305-313).  All pairs run together in batched launches; the
  * main seed bank follows the reference's batch (serial) semantics, i.e.
  * early_term_score applies (search.h:83).  Returns 0, or -2 on a device
- * error.  Pairs with length <= 0 are left untouched (the reference ignored
- * per-pair return codes).  Pairs over the 10239 native-valid residue limit
+ * error (every pair with length > 0 then holds score NaN and the identity,
+ * visible to callers that ignore the return code, as the reference's batch
+ * callers do).  Pairs with length <= 0 are left untouched (the reference
+ * ignored per-pair return codes).  Pairs over the 10239 native-valid residue limit
  * (see indialign_tmscore_search) get score NaN and the identity while the
  * others are searched; the call then returns -6. */
 int indialign_tmscore_search_batch(int n_pairs, const NativePairInput *inputs,

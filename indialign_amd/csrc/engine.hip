@@ -1881,6 +1881,18 @@ Trainer &trainer() {
     return *g_trainer;
 }
 
+// After a device error: score NaN and the identity for every pair the call
+// would have written (length > 0), so a caller that ignores the return code
+// (the reference's batch callers do) cannot read stale or uninitialised
+// results as scores.
+void fail_outputs(const NativePairInput *inp, int n, NativePairResult *out) {
+    for (int k = 0; k < n; k++) {
+        if (inp && inp[k].length <= 0) continue;
+        out[k].score = std::nan("");
+        ia::set_identity(out[k].rotation, out[k].translation);
+    }
+}
+
 int run_search(int n, const NativePairInput *inp, const NativeConfig *cfg, NativePairResult *out,
                int serial, int method = 0, int omp_threads = 1) {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -1896,6 +1908,7 @@ int run_search(int n, const NativePairInput *inp, const NativeConfig *cfg, Nativ
     } catch (const std::exception &e) {
         std::fprintf(stderr, "indialign_amd: %s\n", e.what());
         drop_engine();
+        fail_outputs(inp, n, out);
         return -2;
     }
 }
@@ -1965,6 +1978,7 @@ int indialign_batch_run(int n_pairs, const NativeConfig *cfg, NativePairResult *
     } catch (const std::exception &e) {
         std::fprintf(stderr, "indialign_amd: %s\n", e.what());
         drop_engine();
+        fail_outputs(nullptr, n_pairs, outputs);
         return -2;
     }
 }

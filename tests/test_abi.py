@@ -92,3 +92,30 @@ def test_struct_layout_matches_reference():
     assert ctypes.sizeof(ia.NativeConfig) == 56 and ia.NativeConfig.early_term_score.offset == 48
     assert ctypes.sizeof(ia.NativePairInput) == 56 and ia.NativePairInput.lnorm.offset == 48
     assert ctypes.sizeof(ia.NativePairResult) == 104 and ia.NativePairResult.translation.offset == 80
+
+
+def test_device_error_marks_outputs():
+    """Without a usable GPU the search returns -2 and every pair with
+    length > 0 reads score NaN and the identity (callers that ignore the
+    return code, as the reference's batch callers do, see the failure);
+    pairs with length <= 0 are left untouched.  Runs where no GPU is visible."""
+    if ia.gpu_available():
+        pytest.skip("a GPU is visible: the search would run")
+    import numpy as np
+    lib = ia.load_library()
+    cfg = ia.default_config()
+    p = np.zeros(30)
+    m = np.ones(10, np.uint8)
+    inps = (ia.NativePairInput * 2)()
+    for k, n in enumerate((10, 0)):
+        inps[k] = ia.NativePairInput(n, p.ctypes.data_as(ia.c_double_p), p.ctypes.data_as(ia.c_double_p),
+                                     m.ctypes.data_as(ia.c_u8_p), m.ctypes.data_as(ia.c_u8_p),
+                                     m.ctypes.data_as(ia.c_u8_p), 10.0)
+    res = (ia.NativePairResult * 2)()
+    res[0].score = res[1].score = 42.0
+    assert lib.indialign_tmscore_search_batch(2, inps, ctypes.byref(cfg), res) == -2
+    assert np.isnan(res[0].score) and list(res[0].rotation) == [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    assert res[1].score == 42.0
+    r = ia.NativePairResult()
+    assert lib.indialign_tmscore_search(ctypes.byref(inps[0]), ctypes.byref(cfg), ctypes.byref(r)) == -2
+    assert np.isnan(r.score)
