@@ -172,6 +172,13 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     }
     V rH = 0, rG = 0;
     int st = 0;
+    // FP64 TM chains with the columns in LDS, steps between a round's first
+    // 64 and its last: the next column's operands are read at a running byte
+    // offset from colv (clamped to the last column) instead of re-deriving
+    // the index from q and the lane (0.6% of the chain kernel's time).
+    constexpr bool OPT_C = !F32 && CL && SMODE == SM_TM;
+    int cbyte = 0;
+    const int cbyte_max = 3 * (int)sizeof(V) * (nlen - 1);
 
     // store the word of steps (st & ~15) .. st, the last nsteps codes
     auto flush = [&](int nsteps) {
@@ -184,6 +191,7 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     auto step = [&](auto phase_a, auto general_col, int q, int q_next) {
         constexpr bool PA = decltype(phase_a)::value;
         constexpr bool GC = decltype(general_col)::value;
+        constexpr bool PB = !PA && !GC;
         const V hu = shr1_old(H, rH);
         const V gu = shr1_old(G, rG);
         if constexpr (F32) {
@@ -212,6 +220,10 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         if constexpr (F32) {
             if (kXYZ) { c0x = colx(rq); c0y = coly(rq); c0z = colz(rq); }
             if (kSS) c0s = cols(rq);
+        } else if constexpr (OPT_C && PB) {
+            cbyte += 3 * (int)sizeof(V);
+            const V *cp = (const V *)((const char *)colv + min(cbyte, cbyte_max));
+            nx = cp[0]; ny = cp[1]; nz = cp[2];
         } else {
             int bn;
             if constexpr (GC) bn = b + 1 >= P ? 0 : min(b + 1, nlen - 1);
@@ -272,9 +284,12 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         const int qa = min(64, qend);
         for (int q = 0; q < qa; q++) step(TA(), TA(), q, q + 1 < qend ? q + 1 : 0);
         // phase B, two steps per iteration (lets loop-carried values
-        // alternate registers instead of being copied back every step)
+        // alternate registers instead of being copied back every step).
+        // Phase B starts at q = 64 (a shorter last round has none), where
+        // lane l's next column is q + 1 - l.
         const int qb = last ? qend : P - 1;
         if (!last) next_row(64 * (m + 1) + lane);
+        if constexpr (OPT_C) cbyte = 3 * (int)sizeof(V) * (qa - lane);
         int q = qa;
         for (; q + 1 < qb; q += 2) {
             step(TB(), TB(), q, q + 1);
