@@ -129,10 +129,12 @@ IA_D double recip_in_range(double y) {
     return fma(fma(-y, r, 1.0), r, r);
 }
 
+// One rotation of the cyclic Jacobi on a: computes (c, s), updates a, and
+// returns false when |a_pq| < 1e-15 skips it (core.h:36-57).
 template <int P, int Q>
-IA_HD void jacobi_rot(double a[4][4], double v[4][4]) {
+IA_HD bool jacobi_rot_a(double a[4][4], double &c, double &s) {
     const double apq = a[P][Q];
-    if (fabs(apq) < 1e-15) return;
+    if (fabs(apq) < 1e-15) return false;
     const double theta = (a[Q][Q] - a[P][P]) / (2.0 * apq);
 #ifdef __HIP_DEVICE_COMPILE__
     double tn;
@@ -142,12 +144,12 @@ IA_HD void jacobi_rot(double a[4][4], double v[4][4]) {
     } else {
         tn = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
     }
-    const double c = recip_in_range(sqrt_in_range(1.0 + tn * tn));
+    c = recip_in_range(sqrt_in_range(1.0 + tn * tn));
 #else
     const double tn = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
-    const double c = 1.0 / sqrt(1.0 + tn * tn);
+    c = 1.0 / sqrt(1.0 + tn * tn);
 #endif
-    const double s = tn * c;
+    s = tn * c;
     a[P][P] -= tn * apq;
     a[Q][Q] += tn * apq;
     a[P][Q] = 0.0;
@@ -161,12 +163,23 @@ IA_HD void jacobi_rot(double a[4][4], double v[4][4]) {
         a[r][Q] = s * xp + c * xq;
         a[Q][r] = a[r][Q];
     }
+    return true;
+}
+
+// The same rotation applied to columns P, Q of one row of V.
+template <int P, int Q>
+IA_HD void jacobi_rot_vrow(double c, double s, double vr[4]) {
+    const double xp = vr[P], xq = vr[Q];
+    vr[P] = c * xp - s * xq;
+    vr[Q] = s * xp + c * xq;
+}
+
+template <int P, int Q>
+IA_HD void jacobi_rot(double a[4][4], double v[4][4]) {
+    double c, s;
+    if (!jacobi_rot_a<P, Q>(a, c, s)) return;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const double xp = v[r][P], xq = v[r][Q];
-        v[r][P] = c * xp - s * xq;
-        v[r][Q] = s * xp + c * xq;
-    }
+    for (int r = 0; r < 4; r++) jacobi_rot_vrow<P, Q>(c, s, v[r]);
 }
 
 IA_HD void jacobi4(double a[4][4], double v[4][4]) {
@@ -184,11 +197,9 @@ IA_HD void jacobi4(double a[4][4], double v[4][4]) {
     }
 }
 
-// Horn key matrix from the 3x3 correlation h (row r = P axis, col c = Q
-// axis), dominant eigenvector (first maximum wins), quaternion -> R
-// (core.h:64-90).
-IA_HD void horn_rotation(const double h[9], double *R) {
-    double f[4][4];
+// Horn key matrix of the 3x3 correlation h (row r = P axis, col c = Q axis),
+// core.h:65-75.
+IA_HD void horn_key(const double h[9], double f[4][4]) {
     f[0][0] = h[0] + h[4] + h[8];
     f[0][1] = f[1][0] = h[5] - h[7];
     f[0][2] = f[2][0] = h[6] - h[2];
@@ -199,18 +210,20 @@ IA_HD void horn_rotation(const double h[9], double *R) {
     f[2][2] = -h[0] + h[4] - h[8];
     f[2][3] = f[3][2] = h[5] + h[7];
     f[3][3] = -h[0] - h[4] + h[8];
-    double v[4][4];
-    jacobi4(f, v);
+}
+
+// Index of the largest diagonal entry, first maximum wins (core.h:78-80).
+IA_HD int top_eigen(const double f[4][4]) {
     int top = 0;
     double best = f[0][0];
     if (f[1][1] > best) { top = 1; best = f[1][1]; }
     if (f[2][2] > best) { top = 2; best = f[2][2]; }
     if (f[3][3] > best) { top = 3; best = f[3][3]; }
-    double q0, q1, q2, q3;
-    if (top == 0) { q0 = v[0][0]; q1 = v[1][0]; q2 = v[2][0]; q3 = v[3][0]; }
-    else if (top == 1) { q0 = v[0][1]; q1 = v[1][1]; q2 = v[2][1]; q3 = v[3][1]; }
-    else if (top == 2) { q0 = v[0][2]; q1 = v[1][2]; q2 = v[2][2]; q3 = v[3][2]; }
-    else { q0 = v[0][3]; q1 = v[1][3]; q2 = v[2][3]; q3 = v[3][3]; }
+    return top;
+}
+
+// Normalised quaternion -> R, identity below norm 1e-12 (core.h:81-89).
+IA_HD void quat_to_R(double q0, double q1, double q2, double q3, double *R) {
     const double nrm = sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
     if (nrm < 1e-12) {
         q0 = 1.0; q1 = q2 = q3 = 0.0;
@@ -226,6 +239,22 @@ IA_HD void horn_rotation(const double h[9], double *R) {
     R[6] = 2 * (q1 * q3 + q0 * q2);
     R[7] = 2 * (q2 * q3 - q0 * q1);
     R[8] = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+}
+
+// Horn key matrix from the 3x3 correlation h, dominant eigenvector (first
+// maximum wins), quaternion -> R (core.h:64-90).
+IA_HD void horn_rotation(const double h[9], double *R) {
+    double f[4][4];
+    horn_key(h, f);
+    double v[4][4];
+    jacobi4(f, v);
+    const int top = top_eigen(f);
+    double q0, q1, q2, q3;
+    if (top == 0) { q0 = v[0][0]; q1 = v[1][0]; q2 = v[2][0]; q3 = v[3][0]; }
+    else if (top == 1) { q0 = v[0][1]; q1 = v[1][1]; q2 = v[2][1]; q3 = v[3][1]; }
+    else if (top == 2) { q0 = v[0][2]; q1 = v[1][2]; q2 = v[2][2]; q3 = v[3][2]; }
+    else { q0 = v[0][3]; q1 = v[1][3]; q2 = v[2][3]; q3 = v[3][3]; }
+    quat_to_R(q0, q1, q2, q3, R);
 }
 
 // 3x3 minor of a 4x4 matrix without row I and column J.

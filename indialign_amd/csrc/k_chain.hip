@@ -58,6 +58,16 @@ IA_D float shr1_old(float v, float old) {
     return __int_as_float(shr1_old(__float_as_int(v), __float_as_int(old)));
 }
 
+// 1.0 or 0.0 from a condition by one 32-bit select (the high word; the low
+// word of both is zero).
+template <class V>
+IA_D V cut_weight(bool c) {
+    if constexpr (sizeof(V) == 8) return __hiloint2double(c ? 0x3FF00000 : 0, 0);
+    else return __int_as_float(c ? 0x3F800000 : 0);
+}
+IA_D double fma_v(double a, double b, double c) { return fma(a, b, c); }
+IA_D float fma_v(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // max of two finite values as one v_max instruction.  fmax() would first
 // quiet possible signalling NaNs of operands that arrive through lane moves
 // (an extra v_max per operand); the fill never holds a NaN.
@@ -94,6 +104,7 @@ IA_D float vmax(float a, float b) {
 // G = H + gap_open for diagonally entered cells, else H, is what down and
 // right neighbours add.
 IA_HD int fill_period(int nlen) { return nlen > 65 ? nlen : 65; }
+enum StepKind { ST_A, ST_G, ST_B, ST_F };
 IA_HD int fill_row_start(int a, int P) { return (a >> 6) * P + (a & 63); }
 // Steps of a plen x nlen fill, and its trace words: word (s >> 4) * 64 + l
 // holds lane l's codes of steps (s & ~15) .. (s | 15).
@@ -185,13 +196,19 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         trace[(long long)(st >> 4) * 64 + lane] = nsteps == 16 ? tw : tw >> (32 - 2 * nsteps);
     };
 
-    // PA: phase A (lane q starts its next row).  GC: the general next-column
-    // rule; without it (phase B, not a round's last step) lane l's next
-    // column is q + 1 - l.
-    auto step = [&](auto phase_a, auto general_col, int q, int q_next) {
-        constexpr bool PA = decltype(phase_a)::value;
-        constexpr bool GC = decltype(general_col)::value;
-        constexpr bool PB = !PA && !GC;
+    // Step kinds.  ST_A: phase A (lane q starts its next row; general
+    // next-column rule).  ST_G: a round's last step (general rule).  ST_B:
+    // phase B, lane l's next column is q + 1 - l, clamped to the last column.
+    // ST_F: phase B of a round that is not the last, with nlen >= 65 (P =
+    // nlen): every lane's row is below plen and its column q - l lies in
+    // [1, nlen - 2], so every cell is active, no index needs a clamp, and
+    // the column counter b is set once after the loop instead of per step.
+    auto step = [&](auto kind, int q, int q_next) {
+        constexpr int K = decltype(kind)::value;
+        constexpr bool PA = K == ST_A;
+        constexpr bool GC = K == ST_A || K == ST_G;
+        constexpr bool PB = K == ST_B;
+        constexpr bool PF = K == ST_F;
         const V hu = shr1_old(H, rH);
         const V gu = shr1_old(G, rG);
         if constexpr (F32) {
@@ -212,29 +229,31 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
             Dc = e ? (V)0 : D;
             Gl = e ? (V)0 : G;
         }
-        const bool act = (unsigned)a < (unsigned)plen && b < nlen;
+        const bool act = PF || ((unsigned)a < (unsigned)plen && b < nlen);
+        const int bc = PF ? q - lane : b;  // this step's column
         // next step's column operands (prefetch)
-        const int rq = min(q_next, nlen - 1);  // lane 0's next column, clamped
+        const int rq = PF ? q_next : min(q_next, nlen - 1);  // lane 0's next column
         V nx = 0, ny = 0, nz = 0;
         char ns = 0;
         if constexpr (F32) {
             if (kXYZ) { c0x = colx(rq); c0y = coly(rq); c0z = colz(rq); }
             if (kSS) c0s = cols(rq);
-        } else if constexpr (OPT_C && PB) {
+        } else if constexpr (OPT_C && (PB || PF)) {
             cbyte += 3 * (int)sizeof(V);
-            const V *cp = (const V *)((const char *)colv + min(cbyte, cbyte_max));
+            const V *cp = (const V *)((const char *)colv + (PF ? cbyte : min(cbyte, cbyte_max)));
             nx = cp[0]; ny = cp[1]; nz = cp[2];
         } else {
             int bn;
             if constexpr (GC) bn = b + 1 >= P ? 0 : min(b + 1, nlen - 1);
+            else if constexpr (PF) bn = q + 1 - lane;
             else bn = min(q + 1 - lane, nlen - 1);
             if (kXYZ) { nx = colx(bn); ny = coly(bn); nz = colz(bn); }
             if (kSS) ns = cols(bn);
         }
         const V2 rn = rowHG[rq];
-        V s;
+        V s, cut_w = 0;
         if constexpr (SMODE == SM_MAT) {
-            s = act ? (V)smat[(long long)a * nlen + b] : (V)0;
+            s = act ? (V)smat[(long long)a * nlen + bc] : (V)0;
         } else if (SMODE == SM_SS) {
             s = (sp == cs) ? (V)1 : (V)0;
         } else {
@@ -251,21 +270,31 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
                 tm = SMODE == SM_TM ? recip_in_range(y) : 1.0 / y;
             }
             if (SMODE == SM_TM) {
-                s = (d2 <= cut8) ? tm : (V)0;
+                s = tm;
             } else {
                 s = tm;
                 if (sp == cs) s += (V)0.5;
             }
+            if constexpr (SMODE == SM_TM) cut_w = cut_weight<V>(d2 <= cut8);
         }
-        const V diag = Dc + s;
+        // TM mode: diag = Dc + (d2 <= cut8 ? tm : 0) as fma(w, tm, Dc) with
+        // w = 1 or 0 (one rounding; Dc >= +0.0, so w = 0 leaves Dc exactly)
+        V diag;
+        if constexpr (SMODE == SM_TM) diag = fma_v(cut_w, s, Dc);
+        else diag = Dc + s;
         const V mgl = vmax(gu, Gl);
         const V hm = vmax(diag, mgl);
         const bool cD = diag >= mgl;
         const unsigned tn = cD ? 0u : (Gl >= gu ? 2u : 1u);
         const V gn = cD ? diag + gap : mgl;
-        if (lane == 63 && act) rowHG[b] = V2{hm, gn};  // to the next round's lane 0
+        // to the next round's lane 0 (lane 63's column is q - 63 in phase B)
+        if constexpr (PF) {
+            if (lane == 63) rowHG[q - 63] = V2{hm, gn};
+        } else {
+            if (lane == 63 && act) rowHG[b] = V2{hm, gn};
+        }
         if constexpr (SMODE == SM_MAT) {
-            if (act) Hout[(long long)(a + 1) * (nlen + 1) + b + 1] = (double)hm;
+            if (act) Hout[(long long)(a + 1) * (nlen + 1) + bc + 1] = (double)hm;
         }
         tw = __builtin_amdgcn_alignbit(tn, tw, 2);
         D = hu; H = hm; G = gn;
@@ -273,16 +302,18 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         rH = rn.x; rG = rn.y;
         if ((st & 15) == 15) flush(16);
         st++;
-        b++;
+        if constexpr (!PF) b++;
     };
 
-    using TA = std::true_type;
-    using TB = std::false_type;
+    using KA = std::integral_constant<int, ST_A>;
+    using KG = std::integral_constant<int, ST_G>;
+    using KB = std::integral_constant<int, ST_B>;
+    using KF = std::integral_constant<int, ST_F>;
     for (int m = 0; m < M; m++) {
         const bool last = m == M - 1;
         const int qend = last ? lastl + nlen : P;
         const int qa = min(64, qend);
-        for (int q = 0; q < qa; q++) step(TA(), TA(), q, q + 1 < qend ? q + 1 : 0);
+        for (int q = 0; q < qa; q++) step(KA(), q, q + 1 < qend ? q + 1 : 0);
         // phase B, two steps per iteration (lets loop-carried values
         // alternate registers instead of being copied back every step).
         // Phase B starts at q = 64 (a shorter last round has none), where
@@ -291,12 +322,21 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         if (!last) next_row(64 * (m + 1) + lane);
         if constexpr (OPT_C) cbyte = 3 * (int)sizeof(V) * (qa - lane);
         int q = qa;
-        for (; q + 1 < qb; q += 2) {
-            step(TB(), TB(), q, q + 1);
-            step(TB(), TB(), q + 1, q + 2);
+        if (!last && nlen >= 65) {
+            for (; q + 1 < qb; q += 2) {
+                step(KF(), q, q + 1);
+                step(KF(), q + 1, q + 2);
+            }
+            if (q < qb) step(KF(), q, q + 1);
+            b = qb - lane;  // the column of step qb, the next one
+        } else {
+            for (; q + 1 < qb; q += 2) {
+                step(KB(), q, q + 1);
+                step(KB(), q + 1, q + 2);
+            }
+            if (q < qb) step(KB(), q, q + 1);
         }
-        if (q < qb) step(TB(), TB(), q, q + 1);
-        if (!last) step(TB(), TA(), P - 1, 0);
+        if (!last) step(KG(), P - 1, 0);
     }
     if (st & 15) {  // the last, partial word
         st--;
@@ -402,9 +442,11 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
 //
 // The passes over the aligned pairs are latency-bound gathers, so each lane
 // loads four pairs (32-byte AoS records) before using any of them, and the
-// sums take every pair: a pair outside the cutoff adds +0.0.  The sums start
-// at +0.0 and so never hold -0.0, which makes adding +0.0 the same as
-// skipping the pair, bit for bit.
+// sums take every pair through an FMA with a 0/1 weight w: fma(w, v, sum)
+// rounds once, so w = 1 adds v as written and w = 0 adds a signed zero.  The
+// sums start at +0.0 and so never hold -0.0, which makes adding a zero the
+// same as skipping the pair, bit for bit.  The six cutoffs' rotations are
+// solved with V split by rows over each quad (horn_rotation_quad).
 constexpr int kLiteBatch = 4;
 
 template <bool FAST>
@@ -452,13 +494,14 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
                 for (int k = 0; k < kLiteBatch; k++) {
                     const bool sel = B.in[k] &&
                         dist2_moved_fma(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]) <= lim;
-                    const double x = sel ? B.x[k] : 0.0, y = sel ? B.y[k] : 0.0, z = sel ? B.z[k] : 0.0;
-                    const double qa = sel ? B.a[k] : 0.0, qb = sel ? B.b[k] : 0.0, qc = sel ? B.c[k] : 0.0;
+                    const double w = sel ? 1.0 : 0.0;
+                    const double x = w * B.x[k], y = w * B.y[k], z = w * B.z[k];
                     cnt += sel ? 1 : 0;
-                    c[0] += x; c[1] += y; c[2] += z; c[3] += qa; c[4] += qb; c[5] += qc;
-                    h[0] = fma(x, qa, h[0]); h[1] = fma(x, qb, h[1]); h[2] = fma(x, qc, h[2]);
-                    h[3] = fma(y, qa, h[3]); h[4] = fma(y, qb, h[4]); h[5] = fma(y, qc, h[5]);
-                    h[6] = fma(z, qa, h[6]); h[7] = fma(z, qb, h[7]); h[8] = fma(z, qc, h[8]);
+                    c[0] += x; c[1] += y; c[2] += z;
+                    c[3] = fma(w, B.a[k], c[3]); c[4] = fma(w, B.b[k], c[4]); c[5] = fma(w, B.c[k], c[5]);
+                    h[0] = fma(x, B.a[k], h[0]); h[1] = fma(x, B.b[k], h[1]); h[2] = fma(x, B.c[k], h[2]);
+                    h[3] = fma(y, B.a[k], h[3]); h[4] = fma(y, B.b[k], h[4]); h[5] = fma(y, B.c[k], h[5]);
+                    h[6] = fma(z, B.a[k], h[6]); h[7] = fma(z, B.b[k], h[7]); h[8] = fma(z, B.c[k], h[8]);
                 }
             }
             cnt = group8_isum(cnt);
@@ -474,7 +517,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
                 for (int i = 0; i < 3; i++)
 #pragma unroll
                     for (int j = 0; j < 3; j++) h[3 * i + j] -= fc * cp[i] * cq[j];
-                solve_rotation<FAST>(h, R);
+                solve_rotation_quad<FAST>(h, R);
                 finish_t(cp, cq, R, t);
             }
             if (!done) {
@@ -496,8 +539,10 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
                     dist2_moved(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]) <= lim;
                 if (keep_sel) selmask |= (unsigned long long)(sel ? 1 : 0) << ((base >> 3) + k);
                 cnt += sel ? 1 : 0;
-                c[0] += sel ? B.x[k] : 0.0; c[1] += sel ? B.y[k] : 0.0; c[2] += sel ? B.z[k] : 0.0;
-                c[3] += sel ? B.a[k] : 0.0; c[4] += sel ? B.b[k] : 0.0; c[5] += sel ? B.c[k] : 0.0;
+                // fma(1, v, c) = c + v and fma(0, v, c) = c exactly
+                const double w = sel ? 1.0 : 0.0;
+                c[0] = fma(w, B.x[k], c[0]); c[1] = fma(w, B.y[k], c[1]); c[2] = fma(w, B.z[k], c[2]);
+                c[3] = fma(w, B.a[k], c[3]); c[4] = fma(w, B.b[k], c[4]); c[5] = fma(w, B.c[k], c[5]);
             }
         }
         cnt = group8_isum(cnt);
@@ -521,20 +566,21 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
                     else
                         sel = B.in[k] &&
                               dist2_moved(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]) <= lim;
-                    const double d0 = sel ? B.x[k] - cp[0] : 0.0, d1 = sel ? B.y[k] - cp[1] : 0.0;
-                    const double d2 = sel ? B.z[k] - cp[2] : 0.0;
-                    const double e0 = sel ? B.a[k] - cq[0] : 0.0, e1 = sel ? B.b[k] - cq[1] : 0.0;
-                    const double e2 = sel ? B.c[k] - cq[2] : 0.0;
-                    h[0] += d0 * e0; h[1] += d0 * e1; h[2] += d0 * e2;
-                    h[3] += d1 * e0; h[4] += d1 * e1; h[5] += d1 * e2;
-                    h[6] += d2 * e0; h[7] += d2 * e1; h[8] += d2 * e2;
+                    // h += (d e) w: the product rounded as written, then
+                    // added (w = 1) or skipped (w = 0) exactly
+                    const double w = sel ? 1.0 : 0.0;
+                    const double d0 = B.x[k] - cp[0], d1 = B.y[k] - cp[1], d2 = B.z[k] - cp[2];
+                    const double e0 = B.a[k] - cq[0], e1 = B.b[k] - cq[1], e2 = B.c[k] - cq[2];
+                    h[0] = fma(d0 * e0, w, h[0]); h[1] = fma(d0 * e1, w, h[1]); h[2] = fma(d0 * e2, w, h[2]);
+                    h[3] = fma(d1 * e0, w, h[3]); h[4] = fma(d1 * e1, w, h[4]); h[5] = fma(d1 * e2, w, h[5]);
+                    h[6] = fma(d2 * e0, w, h[6]); h[7] = fma(d2 * e1, w, h[7]); h[8] = fma(d2 * e2, w, h[8]);
                 }
             }
         }
 #pragma unroll
         for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
         if (solve) {
-            solve_rotation<FAST>(h, R);
+            solve_rotation_quad<FAST>(h, R);
             finish_t(cp, cq, R, t);
         }
         if (!done) {
@@ -555,7 +601,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
             const double d2 = dist2_moved(R, t, B.x[k], B.y[k], B.z[k], B.a[k], B.b[k], B.c[k]);
             const bool sel = B.in[k] && d2 <= c8;
             const double tm = recip_in_range(1.0 + div_by(d2, d0sq, inv_d0sq));
-            acc += sel ? tm : 0.0;
+            acc = fma(sel ? 1.0 : 0.0, tm, acc);
         }
     }
     acc = group8_sum(acc);
@@ -675,7 +721,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             }
 #pragma unroll
             for (int q = 0; q < 9; q++) h[q] = wave_sum(h[q]);
-            solve_rotation<FAST>(h, R);
+            solve_rotation_quad<FAST>(h, R);
             finish_t(cp, cq, R, t);
         }
     }

@@ -46,6 +46,54 @@ IA_D int group8_isum(int v) {
     return v;
 }
 
+// Lane k's value of each quad (DPP quad_perm broadcast, a VALU move).
+template <int K>
+IA_D double quad_bcast(double v) {
+    constexpr int ctrl = K * 0x55;
+    return __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(v), ctrl, 0xf, 0xf, false),
+                            __builtin_amdgcn_update_dpp(0, __double2loint(v), ctrl, 0xf, 0xf, false));
+}
+
+// horn_rotation for a correlation h held identically by the four lanes of
+// each quad.  The Jacobi sweep is the reference's (same rotations, same
+// roundings); each lane keeps only row (lane & 3) of V, so a rotation
+// updates two entries of V per lane instead of eight, and the top
+// eigenvector's four components are gathered from the quad at the end.
+// Lanes of a quad must take this call together.
+template <int P, int Q>
+IA_D void jacobi_rot_quad(double a[4][4], double vr[4]) {
+    double c, s;
+    if (!jacobi_rot_a<P, Q>(a, c, s)) return;
+    jacobi_rot_vrow<P, Q>(c, s, vr);
+}
+
+IA_D void horn_rotation_quad(const double h[9], double *R) {
+    const int row = threadIdx.x & 3;
+    double f[4][4];
+    horn_key(h, f);
+    double vr[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) vr[c] = c == row ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 8; sweep++) {
+        jacobi_rot_quad<0, 1>(f, vr);
+        jacobi_rot_quad<0, 2>(f, vr);
+        jacobi_rot_quad<0, 3>(f, vr);
+        jacobi_rot_quad<1, 2>(f, vr);
+        jacobi_rot_quad<1, 3>(f, vr);
+        jacobi_rot_quad<2, 3>(f, vr);
+    }
+    const int top = top_eigen(f);
+    const double mine = top == 0 ? vr[0] : top == 1 ? vr[1] : top == 2 ? vr[2] : vr[3];
+    quat_to_R(quad_bcast<0>(mine), quad_bcast<1>(mine), quad_bcast<2>(mine), quad_bcast<3>(mine), R);
+}
+
+// solve_rotation for a correlation held identically by each quad.
+template <bool FAST>
+IA_D void solve_rotation_quad(const double h[9], double *R) {
+    if (FAST && qcp_rotation(h, R)) return;
+    horn_rotation_quad(h, R);
+}
+
 // Wave-parallel two-pass Kabsch on n point pairs produced by get(q, ...).
 // Every lane ends with the same R, t (xor butterflies are order-symmetric).
 template <bool FAST, class Get>
@@ -74,7 +122,7 @@ IA_D void wave_kabsch(int n, Get get, double *R, double *t) {
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) h[k] = wave_sum(h[k]);
-    solve_rotation<FAST>(h, R);
+    solve_rotation_quad<FAST>(h, R);
     finish_t(cp, cq, R, t);
 }
 
