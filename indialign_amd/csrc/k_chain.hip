@@ -286,7 +286,9 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         const V hm = vmax(diag, mgl);
         const bool cD = diag >= mgl;
         const unsigned tn = cD ? 0u : (Gl >= gu ? 2u : 1u);
-        const V gn = cD ? diag + gap : mgl;
+        // G = diag + gap when entered diagonally (then hm = diag), else mgl
+        // (= hm > 0): fma(1, gap, hm) = diag + gap, fma(0, gap, hm) = hm
+        const V gn = fma_v(cut_weight<V>(cD), gap, hm);
         // to the next round's lane 0 (lane 63's column is q - 63 in phase B)
         if constexpr (PF) {
             if (lane == 63) rowHG[q - 63] = V2{hm, gn};
@@ -402,29 +404,29 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
             unsigned wv[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) wv[q] = w0 - q >= 0 ? trace[(long long)(w0 - q) * 64 + lane] : 0u;
-            while (i > 0 && j > 0) {
-                const int s = fill_row_start(i - 1, P) + j - 1;
-                const int kw = w0 - (s >> 4);
-                if (kw > 3) break;
-                const int l = (i - 1) & 63;
-                const unsigned a0 = (unsigned)__builtin_amdgcn_readlane((int)wv[0], l);
-                const unsigned a1 = (unsigned)__builtin_amdgcn_readlane((int)wv[1], l);
-                const unsigned a2 = (unsigned)__builtin_amdgcn_readlane((int)wv[2], l);
-                const unsigned a3 = (unsigned)__builtin_amdgcn_readlane((int)wv[3], l);
-                const unsigned w = kw == 0 ? a0 : kw == 1 ? a1 : kw == 2 ? a2 : a3;
-                const unsigned d = (w >> ((s & 15) * 2)) & 3u;
-                if (d == 0) {
-                    if (lane == 0) {
-                        L.alP[k] = (short)(i - 1);
-                        L.alN[k] = (short)(j - 1);
+            // the path's word index w0 - (s >> 4) never decreases: walk the
+            // cells of word kk, then move on (one readlane per cell)
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int wk = (int)wv[kk];
+                while (i > 0 && j > 0) {
+                    const int s = fill_row_start(i - 1, P) + j - 1;
+                    if (w0 - (s >> 4) != kk) break;
+                    const unsigned w = (unsigned)__builtin_amdgcn_readlane(wk, (i - 1) & 63);
+                    const unsigned d = (w >> ((s & 15) * 2)) & 3u;
+                    if (d == 0) {
+                        if (lane == 0) {
+                            L.alP[k] = (short)(i - 1);
+                            L.alN[k] = (short)(j - 1);
+                        }
+                        k++;
+                        i--;
+                        j--;
+                    } else if (d == 1) {
+                        i--;
+                    } else {
+                        j--;
                     }
-                    k++;
-                    i--;
-                    j--;
-                } else if (d == 1) {
-                    i--;
-                } else {
-                    j--;
                 }
             }
         }
