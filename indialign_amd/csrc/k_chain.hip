@@ -104,7 +104,7 @@ IA_D float vmax(float a, float b) {
 // G = H + gap_open for diagonally entered cells, else H, is what down and
 // right neighbours add.
 IA_HD int fill_period(int nlen) { return nlen > 65 ? nlen : 65; }
-enum StepKind { ST_A, ST_G, ST_B, ST_F };
+enum StepKind { ST_A, ST_AP, ST_G, ST_GP, ST_B, ST_F };
 IA_HD int fill_row_start(int a, int P) { return (a >> 6) * P + (a & 63); }
 // Steps of a plen x nlen fill, and its trace words: word (s >> 4) * 64 + l
 // holds lane l's codes of steps (s & ~15) .. (s | 15).
@@ -191,6 +191,8 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     int cbyte = 0;
     const int cbyte_max = 3 * (int)sizeof(V) * (nlen - 1);
 
+    int rnd = 0;  // the round (M rounds of 64 rows)
+
     // store the word of steps (st & ~15) .. st, the last nsteps codes
     auto flush = [&](int nsteps) {
         trace[(long long)(st >> 4) * 64 + lane] = nsteps == 16 ? tw : tw >> (32 - 2 * nsteps);
@@ -203,10 +205,17 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     // nlen): every lane's row is below plen and its column q - l lies in
     // [1, nlen - 2], so every cell is active, no index needs a clamp, and
     // the column counter b is set once after the loop instead of per step.
+    //
+    // ST_AP, ST_GP: ST_A and ST_G for FP64 TM chains with the columns in LDS
+    // and nlen >= 65 (P = nlen): every lane's next column comes from the
+    // running byte offset cbyte = 24 b (b, the lane's column, in [0, P]), the
+    // LDS copy holding column 0 again at index P, so a lane on its row's last
+    // column reads its next row's first; the row switch resets cbyte.
     auto step = [&](auto kind, int q, int q_next) {
         constexpr int K = decltype(kind)::value;
-        constexpr bool PA = K == ST_A;
+        constexpr bool PA = K == ST_A || K == ST_AP;
         constexpr bool GC = K == ST_A || K == ST_G;
+        constexpr bool PER = K == ST_AP || K == ST_GP;
         constexpr bool PB = K == ST_B;
         constexpr bool PF = K == ST_F;
         const V hu = shr1_old(H, rH);
@@ -219,17 +228,27 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
             }
             if (kSS) cs = (char)shr1_old((int)cs, (int)c0s);
         }
-        V Dc = D, Gl = G;
         if constexpr (PA) {
-            const bool e = lane == q;
-            if (e) {
-                a += 64; b = 0;
+            // lane q starts its next row: its diagonal and left operands are
+            // the H = 0 boundary.  One lane switches per step, so this is a
+            // branch (moves under a one-lane exec mask), not selects in all
+            // 64 lanes; the empty volatile asm keeps it from being if-converted.
+            if (lane == q) {
+                asm volatile("");
+                if constexpr (PER) {
+                    cbyte = 0;  // a and b are not used on this path
+                } else {
+                    a += 64; b = 0;
+                }
                 mx = nmx; my = nmy; mz = nmz; sp = nsp;
+                D = (V)0; G = (V)0;
             }
-            Dc = e ? (V)0 : D;
-            Gl = e ? (V)0 : G;
         }
-        const bool act = PF || ((unsigned)a < (unsigned)plen && b < nlen);
+        const V Dc = D, Gl = G;
+        // per-lane activity, needed only by the matrix test entry: inactive
+        // cells (rows >= plen, columns >= nlen, round 0's lanes before their
+        // first row) never feed an active one and their codes are never read
+        const bool act = PF || SMODE != SM_MAT || ((unsigned)a < (unsigned)plen && b < nlen);
         const int bc = PF ? q - lane : b;  // this step's column
         // next step's column operands (prefetch)
         const int rq = PF ? q_next : min(q_next, nlen - 1);  // lane 0's next column
@@ -238,9 +257,9 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         if constexpr (F32) {
             if (kXYZ) { c0x = colx(rq); c0y = coly(rq); c0z = colz(rq); }
             if (kSS) c0s = cols(rq);
-        } else if constexpr (OPT_C && (PB || PF)) {
+        } else if constexpr (OPT_C && (PB || PF || PER)) {
             cbyte += 3 * (int)sizeof(V);
-            const V *cp = (const V *)((const char *)colv + (PF ? cbyte : min(cbyte, cbyte_max)));
+            const V *cp = (const V *)((const char *)colv + (PB ? min(cbyte, cbyte_max) : cbyte));
             nx = cp[0]; ny = cp[1]; nz = cp[2];
         } else {
             int bn;
@@ -289,11 +308,15 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         // G = diag + gap when entered diagonally (then hm = diag), else mgl
         // (= hm > 0): fma(1, gap, hm) = diag + gap, fma(0, gap, hm) = hm
         const V gn = fma_v(cut_weight<V>(cD), gap, hm);
-        // to the next round's lane 0 (lane 63's column is q - 63 in phase B)
+        // to the next round's lane 0.  Lane 63's cell is wave-uniform: before
+        // it switches rows (q < 63) it is on row 64 rnd - 1 at column
+        // P + q - 63, from then on on row 64 rnd + 63 at column q - 63.
         if constexpr (PF) {
             if (lane == 63) rowHG[q - 63] = V2{hm, gn};
         } else {
-            if (lane == 63 && act) rowHG[b] = V2{hm, gn};
+            const int r63 = q < 63 ? 64 * rnd - 1 : 64 * rnd + 63;
+            const int c63 = q < 63 ? P + q - 63 : q - 63;
+            if (lane == 63 && r63 >= 0 && r63 < plen && c63 < nlen) rowHG[c63] = V2{hm, gn};
         }
         if constexpr (SMODE == SM_MAT) {
             if (act) Hout[(long long)(a + 1) * (nlen + 1) + bc + 1] = (double)hm;
@@ -304,18 +327,39 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         rH = rn.x; rG = rn.y;
         if ((st & 15) == 15) flush(16);
         st++;
-        if constexpr (!PF) b++;
+        if constexpr (!PF && !PER) b++;
     };
 
+    // periodic column offsets: lane l starts on column P - l (round 0's
+    // lanes before their first row count up to it)
+    const bool per = OPT_C && nlen >= 65;
+    if constexpr (OPT_C) cbyte = 3 * (int)sizeof(V) * (P - lane);
     using KA = std::integral_constant<int, ST_A>;
+    using KAP = std::integral_constant<int, ST_AP>;
+    using KGP = std::integral_constant<int, ST_GP>;
     using KG = std::integral_constant<int, ST_G>;
     using KB = std::integral_constant<int, ST_B>;
     using KF = std::integral_constant<int, ST_F>;
     for (int m = 0; m < M; m++) {
+        rnd = m;
         const bool last = m == M - 1;
         const int qend = last ? lastl + nlen : P;
         const int qa = min(64, qend);
-        for (int q = 0; q < qa; q++) step(KA(), q, q + 1 < qend ? q + 1 : 0);
+        if (per) {
+            int q = 0;
+            for (; q + 1 < qa; q += 2) {
+                step(KAP(), q, q + 1 < qend ? q + 1 : 0);
+                step(KAP(), q + 1, q + 2 < qend ? q + 2 : 0);
+            }
+            if (q < qa) step(KAP(), q, q + 1 < qend ? q + 1 : 0);
+        } else {
+            int q = 0;
+            for (; q + 1 < qa; q += 2) {
+                step(KA(), q, q + 1 < qend ? q + 1 : 0);
+                step(KA(), q + 1, q + 2 < qend ? q + 2 : 0);
+            }
+            if (q < qa) step(KA(), q, q + 1 < qend ? q + 1 : 0);
+        }
         // phase B, two steps per iteration (lets loop-carried values
         // alternate registers instead of being copied back every step).
         // Phase B starts at q = 64 (a shorter last round has none), where
@@ -338,7 +382,10 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
             }
             if (q < qb) step(KB(), q, q + 1);
         }
-        if (!last) step(KG(), P - 1, 0);
+        if (!last) {
+            if (per) step(KGP(), P - 1, 0);
+            else step(KG(), P - 1, 0);
+        }
     }
     if (st & 15) {  // the last, partial word
         st--;
@@ -654,6 +701,13 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
         }
         if (SMODE != SM_TM) L.cols[b] = A.ssn[off + A.nmap[off + b]];
     }
+    // column 0 again at index nlen: the fill's running column offset reads
+    // index P = nlen where a lane's row ends and its next row starts
+    if (CL && SMODE == SM_TM && !FAST && lane == 0) {
+        L.colv[3 * pi.nlen] = A.qcx[off];
+        L.colv[3 * pi.nlen + 1] = A.qcy[off];
+        L.colv[3 * pi.nlen + 2] = A.qcz[off];
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     const bool tm = SMODE == SM_TM;
@@ -861,7 +915,7 @@ struct LdsPlan {
         rowHG = o; o += 16 * n;
         alP = rowHG;  // 2 min(plen, nlen) <= 8 nlen bytes
         alN = rowHG + 8 * n;
-        colv = o; o += 24 * nc;
+        colv = o; o += nc ? 24 * (nc + 1) : 0;  // + a copy of column 0 at index nlen
         misc = o; o += 16;
         cols = o; o += (nc + 15) & ~(size_t)15;
         trace = o;
