@@ -108,17 +108,20 @@ int indialign_cuda_available(void);
  * of each pair only the top-m by the ranker installed with
  * indialign_set_ranker are refined; the threading-DP offsets and the seeds
  * of the nested detailed searches are screened the same way, see
- * indialign_set_learned_budgets).  method = 3: the seed search of the
- * reference's separate PyTorch engine (scoring/_tmscore_gpu/search.py:492-761
- * with dp_iter = 0): window seeds without anchor seeds, cutoffs
- * d0_search x {2, 1.5, 1, .75, .5, .25}, exactly max_iter refinement passes
- * per candidate (no convergence break), candidates ranked by the TM-score
- * with the score_d8 cut, then the top-k distance seed, and the reported
- * score max(TM-score without the cut, ranking score); FP64.  Its DP
- * refinement and cross-index rescue are not rebuilt: method 3 returns -7
- * when dp_iter > 0 or a pair has pred_valid != native_valid.  Same return
- * codes as the batch entry, -3 for an unknown method, -4 for method 2
- * without a ranker. */
+ * indialign_set_learned_budgets).  method = 3: the search of the reference's
+ * separate PyTorch engine with same-index masks
+ * (scoring/_tmscore_gpu/search.py:492-761), each pair as a batch of one:
+ * window seeds without anchor seeds, cutoffs d0_search x {2, 1.5, 1, .75,
+ * .5, .25}, exactly max_iter refinement passes per candidate (no
+ * convergence break), candidates ranked by the TM-score with the score_d8
+ * cut, then the top-k distance seed; with dp_iter > 0 the engine's DP
+ * refinement (dp.py:186-252) from the best seed; the reported score is the
+ * maximum of the DP score, the TM-scores without the cut of the DP and seed
+ * transforms and the ranking score, with the first transform that reaches
+ * it; FP64.  Its cross-index rescue is not rebuilt: method 3 returns -7
+ * when a pair has pred_valid != native_valid.  Same return codes as the
+ * batch entry, -3 for an unknown method, -4 for method 2 without a
+ * ranker. */
 int indialign_search_ex(int n_pairs, const NativePairInput *inputs, const NativeConfig *cfg,
                         NativePairResult *outputs, int serial_bank, int method);
 

@@ -192,8 +192,8 @@ def tmscore_search(pred, native, valid=None, pred_valid=None, native_valid=None,
 
 
 METHOD_PLAIN, METHOD_FAST, METHOD_LEARNED = 0, 1, 2
-# the seed search of the reference's separate PyTorch engine (dp_iter = 0,
-# same-index masks; indialign_search_ex method 3)
+# the search of the reference's separate PyTorch engine (same-index masks;
+# indialign_search_ex method 3)
 METHOD_TORCH_SEED = 3
 # return code of a search with a pair whose compact native length exceeds
 # the NW chain kernel's limit (that pair's score is NaN, the others are set)
@@ -222,8 +222,7 @@ def tmscore_search_batch(pairs, cfg: NativeConfig | None = None, serial_bank: in
         warnings.warn("some pairs have more native-valid residues than the NW chain kernel "
                       "holds; their scores are NaN", RuntimeWarning)
     elif rc == UNSUPPORTED:
-        raise IndialignError("method 3 (the PyTorch engine's seed search) takes dp_iter = 0 and "
-                             "same-index masks only")
+        raise IndialignError("method 3 (the PyTorch engine's search) takes same-index masks only")
     elif rc != 0:
         raise IndialignError(f"batch search returned {rc}")
     return _results(res, pk.n)

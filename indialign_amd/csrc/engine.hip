@@ -97,7 +97,7 @@ constexpr double kNever = 1e300;  // early termination disabled
 
 constexpr int kNStats = 11;
 constexpr int kTooLong = -6;      // return code: some pair exceeds chain_max_nlen()
-constexpr int kUnsupported = -7;  // return code: method 3 with dp_iter > 0 or separate masks
+constexpr int kUnsupported = -7;  // return code: method 3 with separate pred/native masks
 constexpr int kChainClasses = 4;  // compact native length <= 160, 400, 900, longer
 
 class Engine {
@@ -264,6 +264,28 @@ class Engine {
         for (int c = 0; c < 6; c++) b.cut[c] = hp_[pair].info.d0s * mult[c];
         return b;
     }
+    // The PyTorch engine's detailed search over a DP list (dp.py:133-183):
+    // window seeds over the padded list length N with lengths from the
+    // list's valid count list_na[nv_idx], six cutoffs d0_search x mult,
+    // max_iter = 20 fixed passes, candidates ranked with the score_d8 cut.
+    BankDesc torch_list_bank(int pair, int slot, int nv_idx, long long out_idx) {
+        BankDesc b = list_bank(pair, slot, 1, hp_[pair].info.sd8, out_idx);
+        b.torch_list = 1;
+        b.n = hp_[pair].N;
+        b.a1 = nv_idx;
+        b.fixed_iters = 1;
+        b.et = kNever;
+        // the most seeds any valid count of a list of this length yields
+        int kmax = 1;
+        for (int vl = 0; vl <= b.n; vl++) kmax = std::max(kmax, torch_list_seed_count(b.n, vl, b.max_starts));
+        b.cand_cap = kmax * b.C;
+        return b;
+    }
+    // dp_refine of the PyTorch engine (dp.py:186-252) from start, iters
+    // iterations, same-index masks; returns the best (R, t, score), score 0
+    // and the start transform when no iteration scores above 0.
+    std::vector<Hit> torch_dp_refine(const std::vector<int> &pairs, const std::vector<Hit> &start,
+                                     int iters);
     void profile(const char *stage, std::chrono::steady_clock::time_point t0);
 
     // arena
@@ -318,6 +340,7 @@ class Engine {
     void stage_topk();
     void stage_dp();
     void stage_weighted();
+    void torch_final();
 
     // Evaluate banks; returns device results indexed by out_idx (n_out).
     // screen > 0: first score every seed's initial superposition (one
@@ -668,14 +691,16 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     const int ncls = std::max(1, std::min(chain_buckets, kChainClasses));
     auto cls_of = [&](const ChainGroup &g) {
         if (ncls == 1) return 0;
-        const int n = hp_[g.pair].nlen;
+        const int n = g.smat == SM_TORCH ? hp_[g.pair].N : hp_[g.pair].nlen;
         int c = 0;
         while (c < kChainClasses - 1 && n > kBound[c]) c++;
         return std::min(c, ncls - 1);
     };
+    // a chain's matrix: compact plen x nlen, or N x N for SM_TORCH chains
+    auto rows_of = [&](const ChainGroup &g) { return g.smat == SM_TORCH ? hp_[g.pair].N : hp_[g.pair].plen; };
+    auto cols_of = [&](const ChainGroup &g) { return g.smat == SM_TORCH ? hp_[g.pair].N : hp_[g.pair].nlen; };
     auto chain_cost = [&](const ChainGroup &g) {
-        const HostPair &h = hp_[g.pair];
-        return (double)h.plen * h.nlen * g.iters;
+        return (double)rows_of(g) * cols_of(g) * g.iters;
     };
     std::vector<int> order(groups.size());
     for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
@@ -704,15 +729,14 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         queued.push_back(g);
         c.total += g.n_chains;
         total_all += g.n_chains;
-        const HostPair &h = hp_[g.pair];
-        c.mp = std::max(c.mp, h.plen);
-        c.mn = std::max(c.mn, h.nlen);
+        c.mp = std::max(c.mp, rows_of(g));
+        c.mn = std::max(c.mn, cols_of(g));
         long long filled = g.n_chains;  // chains that run the NW fill
         if (g.init == CI_OFFSET && g.k0 <= 0 && (-g.k0) % g.kstep == 0 &&
             (-g.k0) / g.kstep < g.n_chains / 2)
             filled -= 2;  // offset k = 0 is skipped (indialign.cpp:236)
         stats[3] += g.n_chains;
-        stats[2] += (double)h.plen * h.nlen * (double)filled * g.iters;
+        stats[2] += (double)rows_of(g) * cols_of(g) * (double)filled * g.iters;
     }
     if (total_all == 0) return 0;
     // LDS layout per class: the trace and the staged columns each go to LDS
@@ -943,6 +967,62 @@ std::vector<Hit> Engine::dp_refine(const std::vector<int> &pairs, const std::vec
     return res;
 }
 
+
+std::vector<Hit> Engine::torch_dp_refine(const std::vector<int> &pairs,
+                                         const std::vector<Hit> &start, int iters) {
+    const size_t n = pairs.size();
+    std::vector<Hit> best(start), cur(start);
+    for (Hit &h : best) h.score = 0.0;  // best_score starts at 0 (dp.py:212)
+    if (n == 0 || iters <= 0) return best;
+    int nmax = 1;
+    for (int a : pairs) nmax = std::max(nmax, hp_[a].N);
+    for (int it = 0; it < iters; it++) {
+        // slots 0..n-1: the lists; n..2n-1: their valid counts
+        ensure_lists((int)(2 * n), nmax);
+        std::vector<ChainGroup> groups;
+        for (size_t k = 0; k < n; k++) {
+            ChainGroup g{};
+            g.pair = pairs[k];
+            g.init = CI_XFORM;
+            g.mode = CM_LIST;
+            g.smat = SM_TORCH;
+            g.n_chains = 1;
+            g.iters = 1;
+            g.xf_off = (int)k;
+            g.xf_stride = 1;
+            g.out_off = (long long)k;
+            g.out_stride = 1;
+            g.list_slot0 = (int)k;
+            g.nv_slot0 = (int)(n + k);
+            g.gap_ss = 0.0;  // no gap penalty (dp.py:31-33)
+            g.sd8 = hp_[g.pair].info.sd8;
+            groups.push_back(g);
+        }
+        const Hit *dxf = upload(b_xf, cur);
+        Hit *dout = b_chain_out.get<Hit>(n);
+        run_chains(groups, dxf, dout);
+        std::vector<BankDesc> banks;
+        for (size_t k = 0; k < n; k++)
+            banks.push_back(torch_list_bank(pairs[k], (int)k, (int)(n + k), (long long)k));
+        Hit *ddet = run_banks(banks, {}, n);
+        const std::vector<Hit> det = download(ddet, n);
+        const std::vector<ListOut> lo = download(b_lout.get<ListOut>(n), n);
+        for (size_t k = 0; k < n; k++) {
+            // dp.py:221-247: the detailed search replaces the NW superposition
+            // when it scores higher; a list with fewer than 3 valid entries
+            // changes nothing
+            if (lo[k].nvalid < 3) continue;
+            Hit step;
+            std::memcpy(step.R, lo[k].R, 72);
+            std::memcpy(step.t, lo[k].t, 24);
+            step.score = lo[k].it_score;
+            if (det[k].score > step.score) step = det[k];
+            cur[k] = step;
+            if (step.score > best[k].score) best[k] = step;
+        }
+    }
+    return best;
+}
 
 void Engine::profile(const char *stage, std::chrono::steady_clock::time_point t0) {
     static const char *names[kStages] = {"seed_bank",   "ss",           "gapless_threading",
@@ -1627,6 +1707,35 @@ void Engine::stage_weighted() {
     profile("weighted_refine", t0);
 }
 
+// The PyTorch engine's final score (search.py:690-757): without DP, the best
+// of the TM-score without the d8 cut and the ranking score; with dp_iter >
+// 0, DP refinement from the best seed, then the best of its score, the
+// TM-scores of its and of the seed's transforms, and the ranking score.
+// The reported transform is the first one that reaches the reported score.
+void Engine::torch_final() {
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<Hit> seed;
+    for (int a : live_) seed.push_back(best_[a]);
+    const std::vector<Hit> orig = rescore(live_, seed);
+    const int dp_iter = cfg_->dp_iter;
+    if (dp_iter <= 0) {
+        for (size_t k = 0; k < live_.size(); k++)
+            if (orig[k].score > best_[live_[k]].score) best_[live_[k]].score = orig[k].score;
+        profile("dp_refine", t0);
+        return;
+    }
+    const std::vector<Hit> dp = torch_dp_refine(live_, seed, dp_iter);
+    const std::vector<Hit> dp_id = rescore(live_, dp);
+    for (size_t k = 0; k < live_.size(); k++) {
+        const Hit *cand[4] = {&dp[k], &dp_id[k], &orig[k], &seed[k]};
+        const Hit *pick = cand[0];
+        for (int q = 1; q < 4; q++)
+            if (cand[q]->score > pick->score) pick = cand[q];
+        best_[live_[k]] = *pick;
+    }
+    profile("dp_refine", t0);
+}
+
 void Engine::load(const NativePairInput *inp, int n) {
     n_loaded_ = n;
     status_.assign(n, 0);
@@ -1654,8 +1763,7 @@ int Engine::search(const NativeConfig *cfg, int serial_bank, NativePairResult *o
     fast_ = method == 1 || method == 2;
     learned_ = method == 2;
     torch_sem_ = method == 3;
-    if (torch_sem_) {  // only the engine's seed search (dp_iter = 0, same-index masks) is rebuilt
-        if (cfg->dp_iter > 0) return kUnsupported;
+    if (torch_sem_) {  // the engine's cross-index rescue (separate masks) is not rebuilt
         for (const HostPair &h : hp_)
             if (h.cross) return kUnsupported;
     }
@@ -1680,15 +1788,7 @@ int Engine::search(const NativeConfig *cfg, int serial_bank, NativePairResult *o
         stage_seed_bank();
         if (torch_sem_) {
             stage_topk();
-            // reported score: the best of the TM-score without the d8 cut and
-            // the ranking score (search.py:754-757)
-            const std::vector<Hit> rs = rescore(live_, [&] {
-                std::vector<Hit> v;
-                for (int a : live_) v.push_back(best_[a]);
-                return v;
-            }());
-            for (size_t k = 0; k < live_.size(); k++)
-                best_[live_[k]].score = std::max(best_[live_[k]].score, rs[k].score);
+            torch_final();
             goto done;
         }
         stage_ss();

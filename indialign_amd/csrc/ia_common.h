@@ -489,4 +489,29 @@ IA_HD void implicit_seed(int n, int ms, int k, int &start, int &len) {
     start = 0; len = 0;
 }
 
+// Window seeds of a PyTorch-engine DP list (search.py:47-77 over the list,
+// dp.py:158-159): k = 0 is the whole list, then for each length of
+// window_lengths(vl) (vl = the list's valid count) the start positions over
+// the padded list length nl; seed masks are ANDed with the entries' flags.
+IA_HD int torch_list_seed_count(int nl, int vl, int ms) {
+    int lens[9];
+    const int n_lens = window_lengths(vl, lens);
+    int K = 1;
+    for (int j = 0; j < n_lens; j++) K += window_start_count(nl, lens[j], ms);
+    return K;
+}
+
+IA_HD void torch_list_seed(int nl, int vl, int ms, int k, int &start, int &len) {
+    if (k == 0) { start = 0; len = nl; return; }
+    int lens[9];
+    const int n_lens = window_lengths(vl, lens);
+    int r = k - 1;
+    for (int j = 0; j < n_lens; j++) {
+        const int c = window_start_count(nl, lens[j], ms);
+        if (r < c) { start = window_start(nl, lens[j], ms, r); len = lens[j]; return; }
+        r -= c;
+    }
+    start = 0; len = 0;
+}
+
 }  // namespace ia

@@ -34,13 +34,23 @@ struct BankDesc {
     double cut[kMaxCut];
     int sel_m;           // > 0: only seeds A.isel[sel_off + i], i < min(seeds, sel_m)
     long long sel_off;   //      (a screened subset in increasing seed order)
+    // 1: a traceback list of the PyTorch engine's DP (SM_TORCH chains): its
+    // entries carry a validity flag, window seeds start anywhere in [0, n)
+    // (n = the pair's N, the list's padded length) with lengths from the
+    // list's valid count list_na[a1], and a seed with fewer than 3 valid
+    // entries yields no candidate (search.py:47-122, dp.py:133-183)
+    int torch_list;
 };
 
 // ---- NW chains (one wavefront per chain) ---------------------------------
 enum ChainInit { CI_XFORM = 0, CI_FRAG = 1, CI_OFFSET = 2 };
 enum ChainMode { CM_LITE = 0, CM_LIST = 1 };
 // SM_MAT: score read from a supplied matrix (the NW test entry only)
-enum ScoreMode { SM_TM = 0, SM_SS = 1, SM_IA4 = 2, SM_MAT = 3 };
+// SM_TORCH: the PyTorch engine's DP score matrix (dp.py:9-28) over all N x N
+// residue pairs, d2 = max(|p|^2 + |q|^2 - 2 p.q, 0), 1 / (1 + d2 / d0^2)
+// kept where d2 <= score_d8^2 and both residues are valid, NW with no gap
+// penalty and ties diag, then up, then left (dp.py:31-56)
+enum ScoreMode { SM_TM = 0, SM_SS = 1, SM_IA4 = 2, SM_MAT = 3, SM_TORCH = 4 };
 
 // A group of chains of one pair that share everything but the start.
 //   CI_XFORM:  chain of item i, gap g starts from
@@ -77,6 +87,7 @@ struct ChainGroup {
     double gap_ss;        // gap used by SM_SS / SM_IA4 chains (ss.h:75,144)
     int sel;              // CI_FRAG: 0 = every item, 1 = the items A.isel[sel_off + i]
     long long sel_off;
+    int nv_slot0;         // SM_TORCH CM_LIST: list_na[nv_slot0 + c] = the list's valid count
 };
 
 // Per-chain extra outputs of CM_LIST chains.
@@ -84,7 +95,7 @@ struct ListOut {
     double R[9], t[3];
     double it_score;
     int na;
-    int pad;
+    int nvalid;  // entries whose residues are both valid (= na except SM_TORCH)
 };
 
 // Launchers (defined in the .hip files; all asynchronous on `s`).

@@ -37,20 +37,24 @@ IA_D bool get_pt(const Arena &A, int off, const P8 *lq, int a0, int a1, int i, d
     }
     x = p.x; y = p.y; z = p.z;
     qa = q.x; qb = q.y; qc = q.z;
-    return PTS == PTS_FULL ? p.w != 0.0 : true;
+    // full-index points: p.w is the valid flag; list records: p.w is the
+    // entry's flag (1 except masked entries of PyTorch-engine lists)
+    return PTS == PTS_CWIN ? true : p.w != 0.0;
 }
 
 IA_D int bank_points(const BankDesc &B, const int *list_na) {
     return B.pts == PTS_LIST ? list_na[B.a0] : B.n;
 }
 
-IA_D int bank_seeds(const BankDesc &B, int n) {
-    return B.seeds == SEED_EXPLICIT ? B.K : implicit_seed_count(n, B.max_starts);
+IA_D int bank_seeds(const BankDesc &B, int n, const int *list_na) {
+    if (B.seeds == SEED_EXPLICIT) return B.K;
+    if (B.torch_list) return torch_list_seed_count(B.n, list_na[B.a1], B.max_starts);
+    return implicit_seed_count(n, B.max_starts);
 }
 
 // Seeds that run: all, or the screened subset.
-IA_D int bank_live_seeds(const BankDesc &B, int n) {
-    const int k = bank_seeds(B, n);
+IA_D int bank_live_seeds(const BankDesc &B, int n, const int *list_na) {
+    const int k = bank_seeds(B, n, list_na);
     return B.sel_m > 0 && B.sel_m < k ? B.sel_m : k;
 }
 
@@ -181,7 +185,7 @@ IA_D double candidate_score(const Arena &A, const BankDesc &B, int n, const P8 *
 template <int PTS, bool FAST>
 IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const SeedDesc *seeds,
                         const uint8_t *masks, const P8 *lq, int pass_cap, Hit &out,
-                        ContState &st, int &kops) {
+                        ContState &st, int &kops, const int *list_na) {
     const PairInfo &pi = A.info[B.pair];
     const int off = pi.off;
     const int C = B.C;
@@ -192,7 +196,8 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         sd = seeds[B.seed_off + k];
     } else {
         int s0, len;
-        implicit_seed(n, B.max_starts, k, s0, len);
+        if (B.torch_list) torch_list_seed(B.n, list_na[B.a1], B.max_starts, k, s0, len);
+        else implicit_seed(n, B.max_starts, k, s0, len);
         sd.a = s0;
         sd.b = len;
     }
@@ -244,6 +249,12 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
             else ks.add_centroid(x, y, z, qa, qb, qc);
         }
         if (ks.cnt >= 3.0) kops += 31 * (int)ks.cnt + 3400;
+        if (ks.cnt < 3.0 && B.torch_list) {  // the engine drops such seeds (search.py:119-122)
+            out.score = kNeg;
+            set_identity(out.R, out.t);
+            st.it = 0;
+            return true;
+        }
         if (ks.cnt < 3.0) {
             set_identity(R, t);
         } else if (FAST) {
@@ -304,23 +315,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void b
     bool ran = false;
     if (live) {
         Hit h;
-        const int K = bank_live_seeds(B, n);
+        const int K = bank_live_seeds(B, n, list_na);
         if (j >= K * B.C || n <= 0) {
             h.score = -INFINITY;
             set_identity(h.R, h.t);
         } else if (B.pts == PTS_FULL) {
             ran = true;
             done = run_candidate<PTS_FULL, FAST>(A, B, n, j, seeds, masks, nullptr, pass_cap, h, st,
-                                                 kops);
+                                                 kops, list_na);
         } else if (B.pts == PTS_LIST) {
             ran = true;
             const long long base = (long long)B.a0 * list_cap;
             done = run_candidate<PTS_LIST, FAST>(A, B, n, j, seeds, masks, list_pq + base, pass_cap,
-                                                 h, st, kops);
+                                                 h, st, kops, list_na);
         } else {
             ran = true;
             done = run_candidate<PTS_CWIN, FAST>(A, B, n, j, seeds, masks, nullptr, pass_cap, h, st,
-                                                 kops);
+                                                 kops, list_na);
         }
         cands[B.cand_off + j] = h;
     }
@@ -416,8 +427,11 @@ __global__ __launch_bounds__(256) void list_gather_kernel(Arena A, const BankDes
     const int na = list_na[B.a0];
     for (int k = threadIdx.x; k < na; k += 256) {
         P8 r;
-        r.p = A.p4[off + lists_p[base + k]];
-        r.q = A.q4[off + lists_n[base + k]];
+        const int ip = lists_p[base + k], in = lists_n[base + k];
+        r.p = A.p4[off + ip];
+        r.q = A.q4[off + in];
+        // the entry's flag: PyTorch-engine lists keep masked entries
+        r.p.w = !B.torch_list || (A.v[off + ip] != 0 && A.v[off + in] != 0) ? 1.0 : 0.0;
         list_pq[base + k] = r;
     }
 }
@@ -427,7 +441,7 @@ __global__ void bank_seed_count_kernel(const BankDesc *banks, int n_banks, const
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n_banks) return;
     const int n = bank_points(banks[b], list_na);
-    out[b] = n > 0 ? bank_seeds(banks[b], n) : 0;
+    out[b] = n > 0 ? bank_seeds(banks[b], n, list_na) : 0;
 }
 
 // (score, index) first-wins maximum over a block.
@@ -459,7 +473,8 @@ __global__ __launch_bounds__(256) void bank_reduce_kernel(const BankDesc *banks,
                                                           Hit *out) {
     const BankDesc &B = banks[blockIdx.x];
     const int n = bank_points(B, list_na);
-    const int total = n > 0 ? bank_live_seeds(B, n) * B.C : 0;
+    // never past the bank's own candidate slots
+    const int total = n > 0 ? min(bank_live_seeds(B, n, list_na) * B.C, B.cand_cap) : 0;
     const Hit *cb = cands + B.cand_off;
     // serial early termination: stop after the first seed whose candidates
     // lift the running best to >= et (search.h:83)

@@ -124,7 +124,8 @@ template <int SMODE, class TR, bool CL, class V>
 IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const double *t,
                   double gap_d, double cut8_d, double d0sq_s, ChainLds &L, TR *trace,
                   const double *smat = nullptr, double *Hout = nullptr) {
-    constexpr bool kXYZ = SMODE == SM_TM || SMODE == SM_IA4;  // coordinates scored
+    constexpr bool kXYZ = SMODE == SM_TM || SMODE == SM_IA4 || SMODE == SM_TORCH;  // coordinates scored
+    constexpr bool kTorch = SMODE == SM_TORCH;  // full-index rows and columns
     constexpr bool kSS = SMODE == SM_SS || SMODE == SM_IA4;   // SS codes scored
     const int lane = threadIdx.x & 63;
     const int plen = pi.plen, nlen = pi.nlen, off = pi.off;
@@ -135,9 +136,9 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     // column operands: the wavefront's LDS copy (FP64, or FP32 for the fast
     // method) or, for chains too long to stage, the arena
     const V *colv = (const V *)L.colv;
-    auto colx = [&](int b) -> V { if constexpr (F32) return CL ? colv[3 * b] : A.qfx[off + b]; else return CL ? colv[3 * b] : A.qcx[off + b]; };
-    auto coly = [&](int b) -> V { if constexpr (F32) return CL ? colv[3 * b + 1] : A.qfy[off + b]; else return CL ? colv[3 * b + 1] : A.qcy[off + b]; };
-    auto colz = [&](int b) -> V { if constexpr (F32) return CL ? colv[3 * b + 2] : A.qfz[off + b]; else return CL ? colv[3 * b + 2] : A.qcz[off + b]; };
+    auto colx = [&](int b) -> V { if constexpr (F32) return CL ? colv[3 * b] : A.qfx[off + b]; else return CL ? colv[3 * b] : kTorch ? A.qx[off + b] : A.qcx[off + b]; };
+    auto coly = [&](int b) -> V { if constexpr (F32) return CL ? colv[3 * b + 1] : A.qfy[off + b]; else return CL ? colv[3 * b + 1] : kTorch ? A.qy[off + b] : A.qcy[off + b]; };
+    auto colz = [&](int b) -> V { if constexpr (F32) return CL ? colv[3 * b + 2] : A.qfz[off + b]; else return CL ? colv[3 * b + 2] : kTorch ? A.qz[off + b] : A.qcz[off + b]; };
     auto cols = [&](int b) { return CL ? L.cols[b] : A.ssn[off + A.nmap[off + b]]; };
     const V gap = (V)gap_d, cut8 = (V)cut8_d;
     const double inv_d0sq_d = 1.0 / d0sq_s;
@@ -151,12 +152,22 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
     // transformed coordinates / SS code of the lane's next row
     V nmx = 0, nmy = 0, nmz = 0;
     char nsp = 0;
+    // SM_TORCH: the moved row's |p|^2 and its validity (dp.py:18, 27)
+    double np2 = 0.0, p2 = 0.0;
+    bool nvr = false, vr = false;
     auto next_row = [&](int r) {
         const int rc = min(r, plen - 1);
         if (kXYZ) {
             double ex, ey, ez;
-            xform(R, t, A.pcx[off + rc], A.pcy[off + rc], A.pcz[off + rc], ex, ey, ez);
+            if constexpr (kTorch)
+                xform(R, t, A.px[off + rc], A.py[off + rc], A.pz[off + rc], ex, ey, ez);
+            else
+                xform(R, t, A.pcx[off + rc], A.pcy[off + rc], A.pcz[off + rc], ex, ey, ez);
             nmx = (V)ex; nmy = (V)ey; nmz = (V)ez;
+            if constexpr (kTorch) {
+                np2 = ex * ex + ey * ey + ez * ez;
+                nvr = A.v[off + rc] != 0;
+            }
         }
         if (kSS) nsp = A.ssp[off + A.pmap[off + rc]];
     };
@@ -241,6 +252,7 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
                     a += 64; b = 0;
                 }
                 mx = nmx; my = nmy; mz = nmz; sp = nsp;
+                if constexpr (kTorch) { p2 = np2; vr = nvr; }
                 D = (V)0; G = (V)0;
             }
         }
@@ -275,6 +287,15 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
             s = act ? (V)smat[(long long)a * nlen + bc] : (V)0;
         } else if (SMODE == SM_SS) {
             s = (sp == cs) ? (V)1 : (V)0;
+        } else if (SMODE == SM_TORCH) {
+            // d2 = clamp(p2 + n2 - 2 p.q, 0) in the engine's order; the score
+            // is kept where d2 <= score_d8^2 and row and column are valid
+            const double n2 = cx * cx + cy * cy + cz * cz;
+            const double cr = mx * cx + my * cy + mz * cz;
+            const double d2 = vmax((double)(p2 + n2 - 2.0 * cr), 0.0);
+            s = (V)(1.0 / (1.0 + d2 / d0sq_s));
+            const bool keep = d2 <= cut8 && vr && A.v[off + min(bc, nlen - 1)] != 0;
+            cut_w = cut_weight<V>(keep);
         } else {
             const V dx = mx - cx, dy = my - cy, dz = mz - cz;
             V d2, tm;
@@ -299,12 +320,14 @@ IA_D void nw_fill(const Arena &A, const PairInfo &pi, const double *R, const dou
         // TM mode: diag = Dc + (d2 <= cut8 ? tm : 0) as fma(w, tm, Dc) with
         // w = 1 or 0 (one rounding; Dc >= +0.0, so w = 0 leaves Dc exactly)
         V diag;
-        if constexpr (SMODE == SM_TM) diag = fma_v(cut_w, s, Dc);
+        if constexpr (SMODE == SM_TM || SMODE == SM_TORCH) diag = fma_v(cut_w, s, Dc);
         else diag = Dc + s;
         const V mgl = vmax(gu, Gl);
         const V hm = vmax(diag, mgl);
         const bool cD = diag >= mgl;
-        const unsigned tn = cD ? 0u : (Gl >= gu ? 2u : 1u);
+        // ties: diag, then left, then up (nw.h:29-33); the PyTorch engine's
+        // first maximum of (diag, up, left) takes up before left (dp.py:51)
+        const unsigned tn = cD ? 0u : kTorch ? (gu >= Gl ? 1u : 2u) : (Gl >= gu ? 2u : 1u);
         // G = diag + gap when entered diagonally (then hm = diag), else mgl
         // (= hm > 0): fma(1, gap, hm) = diag + gap, fma(0, gap, hm) = hm
         const V gn = fma_v(cut_weight<V>(cD), gap, hm);
@@ -686,8 +709,15 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
                     int *list_na, int list_cap, ChainLds &L, TR *trace) {
     const int lane = threadIdx.x & 63;
     const int off = pi.off;
-    for (int b = lane; CL && b < pi.nlen; b += 64) {
-        if (SMODE != SM_SS) {
+    // SM_TORCH chains fill all N x N residue pairs (full indices)
+    PairInfo pf = pi;
+    if (SMODE == SM_TORCH) pf.plen = pf.nlen = pi.N;
+    for (int b = lane; CL && b < pf.nlen; b += 64) {
+        if (SMODE == SM_TORCH) {
+            L.colv[3 * b] = A.qx[off + b];
+            L.colv[3 * b + 1] = A.qy[off + b];
+            L.colv[3 * b + 2] = A.qz[off + b];
+        } else if (SMODE != SM_SS) {
             if (FAST) {
                 float *cf = (float *)L.colv;
                 cf[3 * b] = A.qfx[off + b];
@@ -699,7 +729,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
                 L.colv[3 * b + 2] = A.qcz[off + b];
             }
         }
-        if (SMODE != SM_TM) L.cols[b] = A.ssn[off + A.nmap[off + b]];
+        if (SMODE == SM_SS || SMODE == SM_IA4) L.cols[b] = A.ssn[off + A.nmap[off + b]];
     }
     // column 0 again at index nlen: the fill's running column offset reads
     // index P = nlen where a lane's row ends and its next row starts
@@ -792,6 +822,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     const double d0dp = fmax(pi.d0, 1.0);
     double d0sq_s;
     if (SMODE == SM_TM) d0sq_s = fmax(d0dp * d0dp, 1e-12);
+    else if (SMODE == SM_TORCH) d0sq_s = fmax(pi.d0 * pi.d0, 1e-12);  // d0, not max(d0, 1)
     else {
         const double d01 = fmax(pi.d0 + 1.5, pi.d0s);  // ss.h:118-119
         d0sq_s = d01 * d01;
@@ -799,7 +830,55 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     const double cut8 = G.sd8 * G.sd8;
     const double d0sq = fmax(pi.d0 * pi.d0, 1e-12);
     for (int it = 0; it < G.iters; it++) {
-        const int na = nw_align<SMODE, TR, CL, FAST>(A, pi, R, t, gap, cut8, d0sq_s, L, trace);
+        const int na = nw_align<SMODE, TR, CL, FAST>(A, pf, R, t, gap, cut8, d0sq_s, L, trace);
+        if (SMODE == SM_TORCH) {
+            // dp.py:189-205: the list keeps every diagonal step; entries
+            // whose residues are not both valid are masked out of the
+            // Kabsch, the iteration score and the detailed search
+            const int slot = G.list_slot0 + c;
+            auto entry = [&](int q, double &x, double &y, double &z, double &a, double &b,
+                             double &cc) -> bool {
+                const int ap = L.alP[q], an = L.alN[q];
+                x = A.px[off + ap]; y = A.py[off + ap]; z = A.pz[off + ap];
+                a = A.qx[off + an]; b = A.qy[off + an]; cc = A.qz[off + an];
+                return A.v[off + ap] != 0 && A.v[off + an] != 0;
+            };
+            int nv = 0;
+            for (int q = lane; q < na; q += 64) nv += (A.v[off + L.alP[q]] != 0 && A.v[off + L.alN[q]] != 0) ? 1 : 0;
+            for (int m = 32; m > 0; m >>= 1) nv += __shfl_xor(nv, m, 64);
+            double nR[9], nt[3];
+            double sc = 0.0;
+            if (nv >= 3) {
+                wave_kabsch_masked<FAST>(na, entry, nR, nt);
+                double acc = 0.0;
+                for (int q = lane; q < na; q += 64) {
+                    double x, y, z, a, b, cc;
+                    const bool in = entry(q, x, y, z, a, b, cc);
+                    acc += in ? tm_term(dist2_moved(nR, nt, x, y, z, a, b, cc), d0sq) : 0.0;
+                }
+                sc = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
+            } else {
+                set_identity(nR, nt);
+            }
+            for (int q = lane; q < na; q += 64) {
+                const int rq = na - 1 - q;
+                lists_p[(long long)slot * list_cap + q] = L.alP[rq];
+                lists_n[(long long)slot * list_cap + q] = L.alN[rq];
+            }
+            if (lane == 0) {
+                list_na[slot] = na;
+                list_na[G.nv_slot0 + c] = nv;
+                ListOut o;
+#pragma unroll
+                for (int q = 0; q < 9; q++) o.R[q] = nR[q];
+                o.t[0] = nt[0]; o.t[1] = nt[1]; o.t[2] = nt[2];
+                o.it_score = sc;
+                o.na = na;
+                o.nvalid = nv;
+                lout[slot] = o;
+            }
+            return;
+        }
         if (G.mode == CM_LIST) {
             const int slot = G.list_slot0 + c;
             double nR[9], nt[3];
@@ -836,7 +915,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
                 o.t[0] = nt[0]; o.t[1] = nt[1]; o.t[2] = nt[2];
                 o.it_score = sc;
                 o.na = na;
-                o.pad = 0;
+                o.nvalid = na;
                 lout[slot] = o;
             }
             return;
@@ -896,7 +975,10 @@ IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, lon
             run_chain<SM_TM, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         else if (G.smat == SM_SS)
             run_chain<SM_SS, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
-        else
+        else if (G.smat == SM_TORCH) {
+            if constexpr (!FAST)
+                run_chain<SM_TORCH, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+        } else
             run_chain<SM_IA4, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         __builtin_amdgcn_wave_barrier();

@@ -126,4 +126,39 @@ IA_D void wave_kabsch(int n, Get get, double *R, double *t) {
     finish_t(cp, cq, R, t);
 }
 
+// wave_kabsch over the entries q < n for which get(q, ...) returns true
+// (a masked point list; at least 3 such entries).  Each entry enters every
+// sum through a 0/1 weight: fma(1, v, s) = s + v, fma(0, v, s) = s.
+template <bool FAST, class Get>
+IA_D void wave_kabsch_masked(int n, Get get, double *R, double *t) {
+    const int lane = threadIdx.x & 63;
+    double c[6] = {0, 0, 0, 0, 0, 0}, cw = 0.0;
+    for (int q = lane; q < n; q += 64) {
+        double x, y, z, a, b, cc;
+        const double w = get(q, x, y, z, a, b, cc) ? 1.0 : 0.0;
+        c[0] = fma(w, x, c[0]); c[1] = fma(w, y, c[1]); c[2] = fma(w, z, c[2]);
+        c[3] = fma(w, a, c[3]); c[4] = fma(w, b, c[4]); c[5] = fma(w, cc, c[5]);
+        cw += w;
+    }
+#pragma unroll
+    for (int d = 0; d < 6; d++) c[d] = wave_sum(c[d]);
+    const double cnt = wave_sum(cw);
+    double cp[3] = {c[0] / cnt, c[1] / cnt, c[2] / cnt};
+    double cq[3] = {c[3] / cnt, c[4] / cnt, c[5] / cnt};
+    double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = lane; q < n; q += 64) {
+        double x, y, z, a, b, cc;
+        const double w = get(q, x, y, z, a, b, cc) ? 1.0 : 0.0;
+        const double d0 = x - cp[0], d1 = y - cp[1], d2 = z - cp[2];
+        const double e0 = a - cq[0], e1 = b - cq[1], e2 = cc - cq[2];
+        h[0] = fma(d0 * e0, w, h[0]); h[1] = fma(d0 * e1, w, h[1]); h[2] = fma(d0 * e2, w, h[2]);
+        h[3] = fma(d1 * e0, w, h[3]); h[4] = fma(d1 * e1, w, h[4]); h[5] = fma(d1 * e2, w, h[5]);
+        h[6] = fma(d2 * e0, w, h[6]); h[7] = fma(d2 * e1, w, h[7]); h[8] = fma(d2 * e2, w, h[8]);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) h[k] = wave_sum(h[k]);
+    solve_rotation_quad<FAST>(h, R);
+    finish_t(cp, cq, R, t);
+}
+
 }  // namespace ia

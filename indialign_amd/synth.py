@@ -156,6 +156,33 @@ def make_pair(seed: int, n: int | None = None, sigma: float | None = None,
                 valid, pv, nv, lnorm, int(seed), float(sigma))
 
 
+def indel_pair(seed: int, n_range=(40, 160), sigma_range=(0.5, 6.0)) -> Pair:
+    """A pair whose residue correspondence is not the identity: pred is the
+    native with k = U{3,10} residues deleted at a and k random-walk residues
+    (5.9 A steps) inserted at b > a, so residues between the two are shifted
+    by k; then the Haar rotation, translation and noise of make_pair.  Same
+    length N on both sides, all masks valid, Lnorm = N."""
+    rng = _rng(seed + 9_000_000)
+    n = int(rng.integers(n_range[0], n_range[1] + 1))
+    sigma = float(rng.uniform(sigma_range[0], sigma_range[1]))
+    native = make_native(n, rng)
+    k = int(rng.integers(3, 11))
+    a = int(rng.integers(n // 8, n // 2))
+    b = int(rng.integers(a + k + 4, n - 2))
+    ins = [native[b - 1]]
+    d = _unit(rng.standard_normal(3))
+    for _ in range(k):
+        d = _turn(d, rng)
+        ins.append(ins[-1] + LOOP_STEP * d)
+    moved = np.concatenate([native[:a], native[a + k:b], np.array(ins[1:]), native[b:]])
+    R = haar_rotation(rng)
+    t = rng.uniform(-50.0, 50.0, size=3)
+    pred = moved @ R.T + t + rng.standard_normal((n, 3)) * sigma
+    m = np.ones(n, np.uint8)
+    return Pair(np.ascontiguousarray(pred), np.ascontiguousarray(native), m, m.copy(), m.copy(),
+                float(n), int(seed), sigma)
+
+
 def single_pair() -> Pair:
     return make_pair(0, n=64, sigma=3.0)
 

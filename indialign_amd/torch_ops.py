@@ -11,16 +11,20 @@ Two semantics are selectable.  semantics="c" (default) runs this library's
 C-ABI search (the reference's indialign_tmscore_search semantics, method 0
 by default, FP64): its stage list, seed schedule and refinement differ from
 the torch engine's, so scores differ from that engine's on the same inputs.
-semantics="torch" runs the torch engine's own seed search
-(scoring/_tmscore_gpu/search.py:492-761 with dp_iter = 0; library method 3):
-window seeds without anchor seeds, cutoffs d0_search x {2, 1.5, 1, .75, .5,
-.25} only, exactly max_iter refinement passes per candidate with no
-convergence break, candidates ranked by the TM-score with the score_d8 cut
-(the engine's _SEARCH_SELECTION_USES_SCORE_D8 default), the top-k distance
-seed, and the reported score max(TM-score without the cut, ranking score).
-The engine's DP refinement and cross-index rescue passes are not rebuilt:
-semantics="torch" with dp_iter > 0 or separate masks raises
-NotImplementedError.  The C search derives d0, d0_search and score_d8 from Lnorm with the
+semantics="torch" runs the torch engine's own search with same-index masks
+(scoring/_tmscore_gpu/search.py:492-761; library method 3): window seeds
+without anchor seeds, cutoffs d0_search x {2, 1.5, 1, .75, .5, .25} only,
+exactly max_iter refinement passes per candidate with no convergence break,
+candidates ranked by the TM-score with the score_d8 cut (the engine's
+_SEARCH_SELECTION_USES_SCORE_D8 default), the top-k distance seed; with
+dp_iter > 0 its DP refinement (dp.py:186-252: NW over all N x N residue
+pairs with masked scores, no gap penalty and up before left on ties, the
+detailed search over the list, dp_iter iterations) from the best seed; the
+reported score is the maximum of the DP score, the TM-scores without the cut
+of the DP and seed transforms, and the ranking score (search.py:690-757).
+Each pair is searched as the engine searches a batch of one.  The engine's
+cross-index rescue (separate pred/native masks) is not rebuilt:
+semantics="torch" with separate masks raises NotImplementedError.  The C search derives d0, d0_search and score_d8 from Lnorm with the
 closed forms of core.h:12-25, which are also the reference's
 d0_from_length (scoring/_tmscore_gpu/runtime.py:91-134); the given tensors
 are checked against them and a different rule raises ValueError, since the
@@ -83,15 +87,12 @@ def tmscore_search(pred, native, valid, d0, d0_search, score_d8, Lnorm, max_iter
     rtol: relative tolerance of the d0 / d0_search / score_d8 check (the
     reference computes them in float32 by default).
     semantics: "c" (the C path, `method` applies) or "torch" (the torch
-    engine's seed search, FP64; dp_iter must be 0 and no separate masks)."""
+    engine's search, FP64; no separate masks)."""
     import torch
 
     if semantics not in ("c", "torch"):
         raise ValueError('semantics must be "c" or "torch"')
     if semantics == "torch":
-        if int(dp_iter) != 0:
-            raise NotImplementedError("semantics='torch': the torch engine's DP refinement "
-                                      "(dp_iter > 0) is not rebuilt")
         for m in (pred_valid, native_valid):
             if m is not None and not bool(torch.equal(m.bool(), valid.bool())):
                 raise NotImplementedError("semantics='torch': the torch engine's cross-index "
