@@ -553,6 +553,10 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
     };
     bool done = !gact;
     int last = -1;
+    // the selection (this lane's bits, aligned sets up to 512 pairs) that
+    // the group's current (R, t) was solved from
+    unsigned long long rset = 0;
+    bool rset_ok = false;
     for (int it = 0; it < 8; it++) {
         if (__all(done)) break;
         double c[6] = {0, 0, 0, 0, 0, 0};
@@ -620,13 +624,17 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
         cnt = group8_isum(cnt);
 #pragma unroll
         for (int d = 0; d < 6; d++) c[d] = group8_sum(c[d]);
-        double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        const bool solve = !done && cnt >= 3;
-        double cp[3], cq[3];
-        const double fc = (double)cnt;
-        cp[0] = c[0] / fc; cp[1] = c[1] / fc; cp[2] = c[2] / fc;
-        cq[0] = c[3] / fc; cq[1] = c[4] / fc; cq[2] = c[5] / fc;
+        // a group whose selection is the set its (R, t) was solved from
+        // would solve the same sums in the same order again: its (R, t)
+        // stands as is (it then stops, its count having repeated)
+        const bool same = group8_isum(keep_sel && rset_ok && selmask == rset ? 0 : 1) == 0;
+        const bool solve = !done && cnt >= 3 && !same;
         if (__any(solve)) {
+            double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            double cp[3], cq[3];
+            const double fc = (double)cnt;
+            cp[0] = c[0] / fc; cp[1] = c[1] / fc; cp[2] = c[2] / fc;
+            cq[0] = c[3] / fc; cq[1] = c[4] / fc; cq[2] = c[5] / fc;
             for (int base = 0; base < na; base += 8 * kLiteBatch) {
                 Batch B;
                 load(base, solve, B);
@@ -648,12 +656,14 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
                     h[6] = fma(d2 * e0, w, h[6]); h[7] = fma(d2 * e1, w, h[7]); h[8] = fma(d2 * e2, w, h[8]);
                 }
             }
-        }
 #pragma unroll
-        for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
-        if (solve) {
-            solve_rotation_quad<FAST>(h, R);
-            finish_t(cp, cq, R, t);
+            for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
+            if (solve) {
+                solve_rotation_quad<FAST>(h, R);
+                finish_t(cp, cq, R, t);
+                rset = selmask;
+                rset_ok = keep_sel;
+            }
         }
         if (!done) {
             if (cnt == last) done = true;
