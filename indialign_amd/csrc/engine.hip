@@ -299,7 +299,7 @@ class Engine {
         b_red_cnt, b_red_out, b_rs_pair, b_rs_in, b_rs_out, b_pairs, b_aoff, b_acnt, b_ascr,
         b_soff, b_fofs, b_fcnt, b_moff, b_feat, b_fgroups, b_pred, b_isel, b_srow, b_sitems,
         b_ssel, b_scnt, b_itemy, b_pmax, b_cont_cnt, b_cont_idx, b_cont_state, b_cblk_bank,
-        b_cblk_first, b_rr_pair, b_rr_out, b_bank_work;
+        b_cblk_first, b_rr_pair, b_rr_out, b_bank_work, b_gq;
 
     std::vector<DBuf *> all_bufs() {
         return {&b_px, &b_py, &b_pz, &b_qx, &b_qy, &b_qz, &b_v, &b_pv, &b_nv, &b_pcx, &b_pcy,
@@ -312,7 +312,8 @@ class Engine {
                 &b_rs_pair, &b_rs_in, &b_rs_out, &b_pairs, &b_aoff, &b_acnt, &b_ascr, &b_soff,
                 &b_fofs, &b_fcnt, &b_moff, &b_feat, &b_fgroups, &b_pred, &b_isel, &b_srow,
                 &b_sitems, &b_ssel, &b_scnt, &b_itemy, &b_pmax, &b_cont_cnt, &b_cont_idx,
-                &b_cont_state, &b_cblk_bank, &b_cblk_first, &b_rr_pair, &b_rr_out, &b_bank_work};
+                &b_cont_state, &b_cblk_bank, &b_cblk_first, &b_rr_pair, &b_rr_out, &b_bank_work,
+                &b_gq};
     }
 
     template <class T>
@@ -713,7 +714,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         long long g0 = 0, ng = 0, total = 0;
         int mp = 1, mn = 1;
         bool in_lds = false, cols_lds = false;
-        long long slots = 0, slot_words = 0, trace_off = 0;
+        long long slots = 0, slot_words = 0, trace_off = 0, gq_off = 0;
         size_t lds = 0;
     };
     std::vector<Cls> cl(kChainClasses);
@@ -748,7 +749,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     int active = 0;
     for (const Cls &c : cl) active += c.total > 0 ? 1 : 0;
     const long long budget = (8LL << 30) / std::max(1, active);  // bytes of global trace slots
-    long long trace_words_all = 0;
+    long long trace_words_all = 0, gq_all = 0;
     for (Cls &c : cl) {
         if (c.total == 0) continue;
         auto occupancy = [&](bool tr, bool co) {
@@ -775,12 +776,15 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         c.slots = std::min<long long>(c.slots, c.total);
         c.trace_off = trace_words_all;
         if (!c.in_lds) trace_words_all += c.slots * c.slot_words;
+        c.gq_off = gq_all;
+        gq_all += c.slots * (c.mn + 32);  // aligned-pair records (na <= nlen) + 32 zeros
     }
     // every buffer is sized before the first launch: the launches of the
     // classes are in flight together
     const ChainGroup *dg = upload(b_groups, queued);
     int *q = b_queue.get<int>(kChainClasses);
     unsigned *trace = trace_words_all ? b_trace.get<unsigned>((size_t)trace_words_all) : nullptr;
+    P8 *gq = b_gq.get<P8>((size_t)std::max(gq_all, 1LL));
     ListOut *lout = b_lout.get<ListOut>(1);
     int *lp = b_lists_p.get<int>(1), *ln = b_lists_n.get<int>(1), *lna = b_list_na.get<int>(1);
     IA_CHECK(hipMemsetAsync(q, 0, sizeof(int) * kChainClasses, s_));
@@ -796,7 +800,8 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         }
         launch_chains(st, A_, dg + c.g0, (int)c.ng, c.total, xf_dev, out_dev, lout, lp, ln, lna,
                       list_cap_, c.mp, c.mn, q + ci, c.in_lds ? nullptr : trace + c.trace_off,
-                      c.slot_words, (int)c.slots, c.cols_lds, fast_, chain_waves);
+                      c.slot_words, (int)c.slots, c.cols_lds, fast_, chain_waves, gq + c.gq_off,
+                      (long long)(c.mn + 32));
         launched();
         IA_CHECK(hipGetLastError());
         if (active > 1) {

@@ -145,7 +145,8 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
-                   int n_slots, bool cols_in_lds, bool fast, int waves);
+                   int n_slots, bool cols_in_lds, bool fast, int waves, P8 *gq,
+                   long long gq_stride);
 // Needleman-Wunsch over supplied score matrices (test entry): problem k has
 // a plen x nlen row-major matrix at smat + s_off, its H and trace codes go
 // to H / T + h_off ((plen + 1) x (nlen + 1), interior only) and its aligned

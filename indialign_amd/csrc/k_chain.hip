@@ -44,7 +44,30 @@ struct ChainLds {
     double *rowHG;    // row hand-over buffer: (H, G) of one row's nlen cells
     short *alP, *alN; // aligned compact indices, reverse order (alias rowHG)
     int *misc;
+    P8 *gq;           // this wavefront's global slot: the aligned pairs as 64-byte records
 };
+
+// Gather the iteration's aligned pairs into the wavefront's slot in list
+// order (record q = compact pred alP[q], native alN[q]), followed by 32 zero
+// records, so the passes over them read linear addresses; the loads past na
+// of the lite search's batches land on the zeros.
+IA_D void gather_aligned(const Arena &A, int off, const ChainLds &L, int na) {
+    const int lane = threadIdx.x & 63;
+    for (int q = lane; q < na + 32; q += 64) {
+        P8 r;
+        if (q < na) {
+            r.p = A.pc4[off + L.alP[q]];
+            r.q = A.qc4[off + L.alN[q]];
+        } else {
+            r.p = P4{0.0, 0.0, 0.0, 0.0};
+            r.q = P4{0.0, 0.0, 0.0, 0.0};
+        }
+        L.gq[q] = r;
+    }
+    // the same wavefront reads the records back (one-wavefront workgroup)
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
 
 // Shift up by one lane (DPP wave_shr:1, a VALU move): lane l receives lane
 // l-1's v, lane 0 receives its own `old`.
@@ -512,14 +535,15 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
 // refinement rounds, TM-score with the d8 cut.  Eight lanes per cutoff; lane
 // s of a group takes aligned pairs s, s+8, s+16, ... in that order.
 //
-// The passes over the aligned pairs are latency-bound gathers, so each lane
-// loads four pairs (32-byte AoS records) before using any of them, and the
-// sums take every pair through an FMA with a 0/1 weight w: fma(w, v, sum)
-// rounds once, so w = 1 adds v as written and w = 0 adds a signed zero.  The
-// sums start at +0.0 and so never hold -0.0, which makes adding a zero the
-// same as skipping the pair, bit for bit.  The six cutoffs' rotations are
-// solved with V split by rows over each quad (horn_rotation_quad).
-constexpr int kLiteBatch = 4;
+// The passes over the aligned pairs read them from the wavefront's slot of
+// gathered 64-byte records (gather_aligned), two pairs per lane before using
+// either, and the sums take every pair through an FMA with a 0/1 weight w:
+// fma(w, v, sum) rounds once, so w = 1 adds v as written and w = 0 adds a
+// signed zero.  The sums start at +0.0 and so never hold -0.0, which makes
+// adding a zero the same as skipping the pair, bit for bit.  The six
+// cutoffs' rotations are solved with V split by rows over each quad
+// (horn_rotation_quad).
+constexpr int kLiteBatch = 2;
 
 template <bool FAST>
 IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int na,
@@ -527,7 +551,6 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
     const int lane = threadIdx.x & 63;
     const int g = lane >> 3, s = lane & 7;
     const bool gact = g < 6;
-    const int off = pi.off;
     const double cut = pi.d0s * d0_mult(g);
     const double lim = cut * cut;
     double R[9], t[3];
@@ -540,15 +563,16 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
         double a[kLiteBatch], b[kLiteBatch], c[kLiteBatch];
         bool in[kLiteBatch];
     };
+    // the gathered records: q < na + 32 stays inside the slot (zeros past na)
+    const P8 *gl = L.gq + s;
     auto load = [&](int base, bool on, Batch &B) {
 #pragma unroll
         for (int k = 0; k < kLiteBatch; k++) {
             const int q = base + s + 8 * k;
             B.in[k] = on && q < na;
-            const int qq = q < na ? q : 0;  // na >= 3
-            const P4 P = A.pc4[off + L.alP[qq]], Q = A.qc4[off + L.alN[qq]];
-            B.x[k] = P.x; B.y[k] = P.y; B.z[k] = P.z;
-            B.a[k] = Q.x; B.b[k] = Q.y; B.c[k] = Q.z;
+            const P8 r = gl[base + 8 * k];
+            B.x[k] = r.p.x; B.y[k] = r.p.y; B.z[k] = r.p.z;
+            B.a[k] = r.q.x; B.b[k] = r.q.y; B.c[k] = r.q.z;
         }
     };
     bool done = !gact;
@@ -931,19 +955,19 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             return;
         }
         if (na < 3) continue;  // align.h:106
+        gather_aligned(A, off, L, na);
         double nR[9], nt[3];
         wave_kabsch<FAST>(na,
                     [&](int q, double &x, double &y, double &z, double &a, double &b, double &cc) {
-                        const int ap = L.alP[q], an = L.alN[q];
-                        x = A.pcx[off + ap]; y = A.pcy[off + ap]; z = A.pcz[off + ap];
-                        a = A.qcx[off + an]; b = A.qcy[off + an]; cc = A.qcz[off + an];
+                        const P8 r = L.gq[q];
+                        x = r.p.x; y = r.p.y; z = r.p.z;
+                        a = r.q.x; b = r.q.y; cc = r.q.z;
                     },
                     nR, nt);
         double acc = 0.0;
         for (int q = lane; q < na; q += 64) {
-            const int ap = L.alP[q], an = L.alN[q];
-            acc += tm_term(dist2_moved(nR, nt, A.pcx[off + ap], A.pcy[off + ap], A.pcz[off + ap],
-                                       A.qcx[off + an], A.qcy[off + an], A.qcz[off + an]), d0sq);
+            const P8 r = L.gq[q];
+            acc += tm_term(dist2_moved(nR, nt, r.p.x, r.p.y, r.p.z, r.q.x, r.q.y, r.q.z), d0sq);
         }
         const double it_score = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
         Hit det;
@@ -1026,7 +1050,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void ch
                                                    int *lists_p, int *lists_n, int *list_na,
                                                    int list_cap, int max_plen, int max_nlen,
                                                    int *queue, unsigned *trace_g,
-                                                   long long slot_words, int cols_in_lds) {
+                                                   long long slot_words, int cols_in_lds,
+                                                   P8 *gq, long long gq_stride) {
     extern __shared__ __align__(16) unsigned char lds[];
     const LdsPlan P(max_plen, max_nlen, trace_g == nullptr, cols_in_lds != 0);
     ChainLds L;
@@ -1037,6 +1062,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void ch
     L.alP = (short *)(lds + P.alP);
     L.alN = (short *)(lds + P.alN);
     L.trace = (unsigned *)(lds + P.trace);
+    L.gq = gq + (long long)blockIdx.x * gq_stride;
     if (trace_g) {
         unsigned *tg = trace_g + (long long)blockIdx.x * slot_words;
         if (cols_in_lds)
@@ -1074,6 +1100,7 @@ __global__ __launch_bounds__(64) void nw_matrix_kernel(const NwProblem *probs, c
     L.alP = (short *)(lds + P.alP);
     L.alN = (short *)(lds + P.alN);
     L.trace = (unsigned *)(lds + P.trace);
+    L.gq = nullptr;
     const int lane = threadIdx.x & 63;
     const NwProblem pr = probs[blockIdx.x];
     PairInfo pi{};
@@ -1162,14 +1189,16 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    long long n_chains, const Hit *xf, Hit *out, ListOut *lout, int *lists_p,
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
-                   int n_slots, bool cols_in_lds, bool fast, int waves) {
+                   int n_slots, bool cols_in_lds, bool fast, int waves, P8 *gq,
+                   long long gq_stride) {
     if (n_chains <= 0) return;
     const size_t lds = chain_lds_bytes(max_plen, max_nlen, trace_scratch == nullptr, cols_in_lds);
     const int grid = n_slots;
 #define IA_LAUNCH_CHAIN(F, W)                                                                    \
     hipLaunchKernelGGL((chain_kernel<F, W>), dim3(grid), dim3(64), lds, s, A, groups, n_groups, \
                        n_chains, xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen,   \
-                       max_nlen, queue, trace_scratch, trace_slot_words, (int)cols_in_lds)
+                       max_nlen, queue, trace_scratch, trace_slot_words, (int)cols_in_lds, gq,  \
+                       gq_stride)
     if (waves == 2) {
         if (fast) IA_LAUNCH_CHAIN(true, 2);
         else IA_LAUNCH_CHAIN(false, 2);
