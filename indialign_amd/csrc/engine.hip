@@ -270,7 +270,7 @@ class Engine {
     // max_iter = 20 fixed passes, candidates ranked with the score_d8 cut.
     BankDesc torch_list_bank(int pair, int slot, int nv_idx, long long out_idx) {
         BankDesc b = list_bank(pair, slot, 1, hp_[pair].info.sd8, out_idx);
-        b.torch_list = 1;
+        b.pts = PTS_TLIST;
         b.n = hp_[pair].N;
         b.a1 = nv_idx;
         b.fixed_iters = 1;
@@ -553,7 +553,7 @@ void Engine::screen_seeds(std::vector<BankDesc> &banks, const SeedDesc *seeds_de
     Hit *dc = b_scands.get<Hit>(cand);
     launch_bank(s_, A_, db, dbb, dbf, (int)blk_bank.size(), seeds_dev, b_masks.get<uint8_t>(1),
                 b_list_pq.get<P8>(1), b_list_na.get<int>(1), list_cap_, dc, fast_, 0, nullptr,
-                nullptr, nullptr);
+                nullptr, nullptr, false);  // screened banks are never PyTorch-engine lists
     launched();
     float *sc = b_pred.get<float>(cand);
     launch_hit_scores(s_, dc, cand, 1, sc);
@@ -581,7 +581,7 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
         // the list copies screen_seeds reads are gathered below as well
         int max_slot = -1;
         for (const BankDesc &b : banks)
-            if (b.pts == PTS_LIST) max_slot = std::max(max_slot, b.a0);
+            if (b.pts == PTS_LIST || b.pts == PTS_TLIST) max_slot = std::max(max_slot, b.a0);
         if (max_slot >= 0) {
             P8 *lpq = b_list_pq.get<P8>((size_t)(max_slot + 1) * list_cap_);
             launch_list_gather(s_, A_, upload(b_banks, banks), (int)banks.size(),
@@ -597,6 +597,7 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
     // own candidates' slots, so the order changes no result.
     std::vector<int> bank_order(banks.size());
     std::vector<double> bank_cost(banks.size());
+    const bool tlist = !banks.empty() && banks[0].pts == PTS_TLIST;
     long long cand = 0;
     for (size_t b = 0; b < banks.size(); b++) {
         banks[b].cand_off = cand;
@@ -604,6 +605,8 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
         stats[1] += banks[b].cand_cap;
         const HostPair &h = hp_[banks[b].pair];
         const int n = banks[b].pts == PTS_LIST ? std::min(h.plen, h.nlen) : banks[b].n;
+        if ((banks[b].pts == PTS_TLIST) != tlist)
+            throw DevError("PyTorch-engine list banks run in a launch of their own");
         bank_cost[b] = (double)n * banks[b].iters;
         bank_order[b] = (int)b;
     }
@@ -624,7 +627,7 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
     Hit *dc = b_cands.get<Hit>(cand);
     int max_slot = -1;
     for (const BankDesc &b : banks)
-        if (b.pts == PTS_LIST) max_slot = std::max(max_slot, b.a0);
+        if (b.pts == PTS_LIST || b.pts == PTS_TLIST) max_slot = std::max(max_slot, b.a0);
     P8 *lpq = b_list_pq.get<P8>((size_t)(max_slot + 1) * list_cap_);
     if (max_slot >= 0 && screen <= 0) {
         launch_list_gather(s_, A_, db, (int)banks.size(), b_lists_p.get<int>(1),
@@ -647,7 +650,7 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
     IA_CHECK(hipEventRecord(e0_, s_));
     launch_bank(s_, A_, db, dbb, dbf, (int)blk_bank.size(), ds, b_masks.get<uint8_t>(1), lpq,
                 b_list_na.get<int>(1), list_cap_, dc, fast_, cont ? bank_pass_cap : 0, ccnt, cidx,
-                cst);
+                cst, tlist);
     launched();
     IA_CHECK(hipGetLastError());
     if (cont) {
@@ -661,7 +664,7 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
         if (!cb.empty()) {
             launch_bank_cont(s_, A_, db, upload(b_cblk_bank, cb), upload(b_cblk_first, cf),
                              (int)cb.size(), lpq, b_list_na.get<int>(1), list_cap_, dc, fast_,
-                             ccnt, cidx, cst);
+                             ccnt, cidx, cst, tlist);
             launched();
             IA_CHECK(hipGetLastError());
         }
@@ -703,6 +706,11 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
     auto chain_cost = [&](const ChainGroup &g) {
         return (double)rows_of(g) * cols_of(g) * g.iters;
     };
+    // SM_TORCH chains run in a kernel instantiation of their own
+    const bool torch = !groups.empty() && groups[0].smat == SM_TORCH;
+    for (const ChainGroup &g : groups)
+        if ((g.smat == SM_TORCH) != torch)
+            throw DevError("PyTorch-engine DP chains run in a launch of their own");
     std::vector<int> order(groups.size());
     for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -754,7 +762,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         if (c.total == 0) continue;
         auto occupancy = [&](bool tr, bool co) {
             const size_t b = chain_lds_bytes(c.mp, c.mn, tr, co);
-            return b > kLdsMax ? 0 : chain_blocks_per_cu(b, fast_, chain_waves);
+            return b > kLdsMax ? 0 : chain_blocks_per_cu(b, fast_, chain_waves, torch);
         };
         const int occ_free = occupancy(false, false);
         if (occ_free <= 0) throw DevError("chain too long for one workgroup's LDS");
@@ -765,7 +773,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
                                       : lds_fits && occupancy(true, false) >= occ_free;
         c.cols_lds = occupancy(c.in_lds, true) >= occupancy(c.in_lds, false);
         c.lds = chain_lds_bytes(c.mp, c.mn, c.in_lds, c.cols_lds);
-        int per_cu = std::max(1, chain_blocks_per_cu(c.lds, fast_, chain_waves));
+        int per_cu = std::max(1, chain_blocks_per_cu(c.lds, fast_, chain_waves, torch));
         per_cu = std::min(per_cu, 16);
         if (chain_slots_per_cu > 0) per_cu = std::min(per_cu, chain_slots_per_cu);
         c.slots = (long long)n_cu_ * per_cu;
@@ -801,7 +809,7 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         launch_chains(st, A_, dg + c.g0, (int)c.ng, c.total, xf_dev, out_dev, lout, lp, ln, lna,
                       list_cap_, c.mp, c.mn, q + ci, c.in_lds ? nullptr : trace + c.trace_off,
                       c.slot_words, (int)c.slots, c.cols_lds, fast_, chain_waves, gq + c.gq_off,
-                      (long long)(c.mn + 32));
+                      (long long)(c.mn + 32), torch);
         launched();
         IA_CHECK(hipGetLastError());
         if (active > 1) {

@@ -6,7 +6,7 @@
 namespace ia {
 
 // ---- seed bank (one lane per candidate) ---------------------------------
-enum PointKind { PTS_FULL = 0, PTS_LIST = 1, PTS_CWIN = 2 };
+enum PointKind { PTS_FULL = 0, PTS_LIST = 1, PTS_CWIN = 2, PTS_TLIST = 3 };
 enum SeedKind { SEED_EXPLICIT = 0, SEED_IMPLICIT = 1 };
 enum InitKind { INIT_SEED = 0, INIT_OFFSET = 1 };
 
@@ -21,6 +21,14 @@ struct SeedDesc {
 //   PTS_LIST: aligned pairs (lp[k], ln[k]) from an NW traceback list slot
 //             (a0 = list slot); the count comes from list_na[a0].
 //   PTS_CWIN: compact window pmap[a0 + k] <-> nmap[a1 + k], k < n.
+//   PTS_TLIST: a traceback list of the PyTorch engine's DP (SM_TORCH
+//             chains), list slot a0: its entries carry a validity flag,
+//             window seeds start anywhere in [0, n) (n = the pair's N, the
+//             list's padded length) with lengths from the list's valid
+//             count list_na[a1], and a seed with fewer than 3 valid entries
+//             yields no candidate (search.py:47-122, dp.py:133-183).
+//             Only the bank kernels' TLIST instantiations run these banks,
+//             and they run nothing else.
 struct BankDesc {
     int pair, pts, n, a0, a1;
     int seeds, seed_off, K, max_starts;
@@ -34,12 +42,6 @@ struct BankDesc {
     double cut[kMaxCut];
     int sel_m;           // > 0: only seeds A.isel[sel_off + i], i < min(seeds, sel_m)
     long long sel_off;   //      (a screened subset in increasing seed order)
-    // 1: a traceback list of the PyTorch engine's DP (SM_TORCH chains): its
-    // entries carry a validity flag, window seeds start anywhere in [0, n)
-    // (n = the pair's N, the list's padded length) with lengths from the
-    // list's valid count list_na[a1], and a seed with fewer than 3 valid
-    // entries yields no candidate (search.py:47-122, dp.py:133-183)
-    int torch_list;
 };
 
 // ---- NW chains (one wavefront per chain) ---------------------------------
@@ -107,13 +109,13 @@ void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int
                  const int *blk_first, int n_blocks, const SeedDesc *seeds,
                  const uint8_t *masks, const P8 *list_pq, const int *list_na, int list_cap,
                  Hit *cands, bool fast, int pass_cap, int *cont_cnt, int *cont_idx,
-                 void *cont_state);
+                 void *cont_state, bool tlist);
 // The rest of those candidates: block b runs entries blk_first[b] .. +255 of
 // bank blk_bank[b]'s list.
 void launch_bank_cont(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
                       const int *blk_first, int n_blocks, const P8 *list_pq, const int *list_na,
                       int list_cap, Hit *cands, bool fast, const int *cont_cnt,
-                      const int *cont_idx, const void *cont_state);
+                      const int *cont_idx, const void *cont_state, bool tlist);
 // Interleaved copies of the traceback lists of the PTS_LIST banks, read by
 // launch_bank (list_pq[slot * list_cap + k]).
 void launch_list_gather(hipStream_t s, const Arena &A, const BankDesc *banks, int n_banks,
@@ -146,7 +148,7 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
                    int n_slots, bool cols_in_lds, bool fast, int waves, P8 *gq,
-                   long long gq_stride);
+                   long long gq_stride, bool torch);
 // Needleman-Wunsch over supplied score matrices (test entry): problem k has
 // a plen x nlen row-major matrix at smat + s_off, its H and trace codes go
 // to H / T + h_off ((plen + 1) x (nlen + 1), interior only) and its aligned
@@ -257,6 +259,6 @@ size_t chain_lds_bytes(int max_plen, int max_nlen, bool trace_in_lds, bool cols_
 // 32-bit trace words of the largest chain fill (max_plen x max_nlen)
 long long chain_trace_words(int max_plen, int max_nlen);
 // resident chain wavefronts per CU for the W = waves variant
-int chain_blocks_per_cu(size_t lds, bool fast, int waves);
+int chain_blocks_per_cu(size_t lds, bool fast, int waves, bool torch);
 
 }  // namespace ia

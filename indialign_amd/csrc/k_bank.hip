@@ -27,7 +27,7 @@ IA_D bool get_pt(const Arena &A, int off, const P8 *lq, int a0, int a1, int i, d
         const P8 r = A.pq8[off + i];
         p = r.p;
         q = r.q;
-    } else if (PTS == PTS_LIST) {
+    } else if (PTS == PTS_LIST || PTS == PTS_TLIST) {
         const P8 r = lq[i];
         p = r.p;
         q = r.q;
@@ -37,18 +37,18 @@ IA_D bool get_pt(const Arena &A, int off, const P8 *lq, int a0, int a1, int i, d
     }
     x = p.x; y = p.y; z = p.z;
     qa = q.x; qb = q.y; qc = q.z;
-    // full-index points: p.w is the valid flag; list records: p.w is the
-    // entry's flag (1 except masked entries of PyTorch-engine lists)
-    return PTS == PTS_CWIN ? true : p.w != 0.0;
+    // full-index points: p.w is the valid flag; PyTorch-engine list records:
+    // p.w is the entry's flag (0 for masked entries)
+    return PTS == PTS_FULL || PTS == PTS_TLIST ? p.w != 0.0 : true;
 }
 
 IA_D int bank_points(const BankDesc &B, const int *list_na) {
-    return B.pts == PTS_LIST ? list_na[B.a0] : B.n;
+    return B.pts == PTS_LIST || B.pts == PTS_TLIST ? list_na[B.a0] : B.n;
 }
 
 IA_D int bank_seeds(const BankDesc &B, int n, const int *list_na) {
     if (B.seeds == SEED_EXPLICIT) return B.K;
-    if (B.torch_list) return torch_list_seed_count(B.n, list_na[B.a1], B.max_starts);
+    if (B.pts == PTS_TLIST) return torch_list_seed_count(B.n, list_na[B.a1], B.max_starts);
     return implicit_seed_count(n, B.max_starts);
 }
 
@@ -196,7 +196,7 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
         sd = seeds[B.seed_off + k];
     } else {
         int s0, len;
-        if (B.torch_list) torch_list_seed(B.n, list_na[B.a1], B.max_starts, k, s0, len);
+        if (PTS == PTS_TLIST) torch_list_seed(B.n, list_na[B.a1], B.max_starts, k, s0, len);
         else implicit_seed(n, B.max_starts, k, s0, len);
         sd.a = s0;
         sd.b = len;
@@ -249,7 +249,7 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
             else ks.add_centroid(x, y, z, qa, qb, qc);
         }
         if (ks.cnt >= 3.0) kops += 31 * (int)ks.cnt + 3400;
-        if (ks.cnt < 3.0 && B.torch_list) {  // the engine drops such seeds (search.py:119-122)
+        if (PTS == PTS_TLIST && ks.cnt < 3.0) {  // the engine drops such seeds (search.py:119-122)
             out.score = kNeg;
             set_identity(out.R, out.t);
             st.it = 0;
@@ -298,7 +298,10 @@ IA_D bool run_candidate(const Arena &A, const BankDesc &B, int n, int j, const S
 // Occupancy (measured): six wavefronts per SIMD for bank_kernel and four for
 // bank_cont_kernel, both methods; the few registers this spills cost less
 // than the latency the extra wavefronts hide.
-template <bool FAST>
+// TL: the instantiation that runs the PyTorch-engine list banks (PTS_TLIST)
+// and nothing else; the others never compile that point kind, which keeps
+// its per-point flag test out of their loops.
+template <bool FAST, bool TL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void bank_kernel(
     Arena A, const BankDesc *banks, const int *blk_bank, const int *blk_first,
     const SeedDesc *seeds, const uint8_t *masks, const P8 *list_pq, const int *list_na,
@@ -319,6 +322,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void b
         if (j >= K * B.C || n <= 0) {
             h.score = -INFINITY;
             set_identity(h.R, h.t);
+        } else if constexpr (TL) {
+            ran = true;
+            const long long base = (long long)B.a0 * list_cap;
+            done = run_candidate<PTS_TLIST, FAST>(A, B, n, j, seeds, masks, list_pq + base,
+                                                  pass_cap, h, st, kops, list_na);
         } else if (B.pts == PTS_FULL) {
             ran = true;
             done = run_candidate<PTS_FULL, FAST>(A, B, n, j, seeds, masks, nullptr, pass_cap, h, st,
@@ -362,7 +370,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void b
 
 // The unfinished candidates of each bank, 256 per block: the rest of their
 // refinement (from their saved state) and their TM-score.
-template <bool FAST>
+template <bool FAST, bool TL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void bank_cont_kernel(Arena A, const BankDesc *banks,
                                                         const int *blk_bank, const int *blk_first,
                                                         const P8 *list_pq, const int *list_na,
@@ -386,7 +394,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void b
         t[0] = h.t[0]; t[1] = h.t[1]; t[2] = h.t[2];
         const int c = j % B.C;
         double score;
-        if (B.pts == PTS_FULL) {
+        if constexpr (TL) {
+            const P8 *lq = list_pq + (long long)B.a0 * list_cap;
+            refine_passes<PTS_TLIST, FAST>(A, B, n, c, lq, R, t, st.it, st.last, B.iters, kops);
+            score = candidate_score<PTS_TLIST, FAST>(A, B, n, lq, R, t);
+        } else if (B.pts == PTS_FULL) {
             refine_passes<PTS_FULL, FAST>(A, B, n, c, nullptr, R, t, st.it, st.last, B.iters, kops);
             score = candidate_score<PTS_FULL, FAST>(A, B, n, nullptr, R, t);
         } else if (B.pts == PTS_LIST) {
@@ -421,7 +433,7 @@ __global__ __launch_bounds__(256) void list_gather_kernel(Arena A, const BankDes
                                                           const int *list_na, int list_cap,
                                                           P8 *list_pq) {
     const BankDesc &B = banks[blockIdx.x];
-    if (B.pts != PTS_LIST) return;
+    if (B.pts != PTS_LIST && B.pts != PTS_TLIST) return;
     const int off = A.info[B.pair].off;
     const long long base = (long long)B.a0 * list_cap;
     const int na = list_na[B.a0];
@@ -431,7 +443,7 @@ __global__ __launch_bounds__(256) void list_gather_kernel(Arena A, const BankDes
         r.p = A.p4[off + ip];
         r.q = A.q4[off + in];
         // the entry's flag: PyTorch-engine lists keep masked entries
-        r.p.w = !B.torch_list || (A.v[off + ip] != 0 && A.v[off + in] != 0) ? 1.0 : 0.0;
+        r.p.w = B.pts != PTS_TLIST || (A.v[off + ip] != 0 && A.v[off + in] != 0) ? 1.0 : 0.0;
         list_pq[base + k] = r;
     }
 }
@@ -645,33 +657,33 @@ void launch_bank(hipStream_t s, const Arena &A, const BankDesc *banks, const int
                  const int *blk_first, int n_blocks, const SeedDesc *seeds,
                  const uint8_t *masks, const P8 *list_pq, const int *list_na, int list_cap,
                  Hit *cands, bool fast, int pass_cap, int *cont_cnt, int *cont_idx,
-                 void *cont_state) {
+                 void *cont_state, bool tlist) {
     if (n_blocks <= 0) return;
     ContState *cs = (ContState *)cont_state;
-    if (fast)
-        bank_kernel<true><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks,
-                                                   list_pq, list_na, list_cap, cands, pass_cap,
-                                                   cont_cnt, cont_idx, cs);
-    else
-        bank_kernel<false><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks,
-                                                    list_pq, list_na, list_cap, cands, pass_cap,
-                                                    cont_cnt, cont_idx, cs);
+#define IA_LAUNCH_BANK(F, T)                                                                    \
+    bank_kernel<F, T><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, seeds, masks,     \
+                                               list_pq, list_na, list_cap, cands, pass_cap,     \
+                                               cont_cnt, cont_idx, cs)
+    if (tlist) IA_LAUNCH_BANK(false, true);  // PyTorch-engine lists: FP64 only
+    else if (fast) IA_LAUNCH_BANK(true, false);
+    else IA_LAUNCH_BANK(false, false);
+#undef IA_LAUNCH_BANK
 }
 
 void launch_bank_cont(hipStream_t s, const Arena &A, const BankDesc *banks, const int *blk_bank,
                       const int *blk_first, int n_blocks, const P8 *list_pq, const int *list_na,
                       int list_cap, Hit *cands, bool fast, const int *cont_cnt,
-                      const int *cont_idx, const void *cont_state) {
+                      const int *cont_idx, const void *cont_state, bool tlist) {
     if (n_blocks <= 0) return;
     const ContState *cs = (const ContState *)cont_state;
-    if (fast)
-        bank_cont_kernel<true><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, list_pq,
-                                                        list_na, list_cap, cands, cont_cnt,
-                                                        cont_idx, cs);
-    else
-        bank_cont_kernel<false><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, list_pq,
-                                                         list_na, list_cap, cands, cont_cnt,
-                                                         cont_idx, cs);
+#define IA_LAUNCH_CONT(F, T)                                                                    \
+    bank_cont_kernel<F, T><<<n_blocks, 256, 0, s>>>(A, banks, blk_bank, blk_first, list_pq,     \
+                                                    list_na, list_cap, cands, cont_cnt,         \
+                                                    cont_idx, cs)
+    if (tlist) IA_LAUNCH_CONT(false, true);
+    else if (fast) IA_LAUNCH_CONT(true, false);
+    else IA_LAUNCH_CONT(false, false);
+#undef IA_LAUNCH_CONT
 }
 
 void launch_list_gather(hipStream_t s, const Arena &A, const BankDesc *banks, int n_banks,

@@ -989,7 +989,9 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
     if (lane == 0) *dst = best;
 }
 
-template <class TR, bool CL, bool FAST>
+// TORCH: the instantiation that runs the PyTorch engine's DP chains
+// (SM_TORCH) and nothing else; the others never compile that score mode.
+template <class TR, bool CL, bool FAST, bool TORCH>
 IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, long long n_chains,
                      const Hit *xf, Hit *out, ListOut *lout, int *lists_p, int *lists_n,
                      int *list_na, int list_cap, int *queue, ChainLds &L, TR *trace) {
@@ -1005,14 +1007,13 @@ IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, lon
         const ChainGroup &G = groups[gi];
         const int c = (int)(task - G.chain_off);
         const PairInfo &pi = A.info[G.pair];
-        if (G.smat == SM_TM)
+        if constexpr (TORCH)
+            run_chain<SM_TORCH, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
+        else if (G.smat == SM_TM)
             run_chain<SM_TM, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         else if (G.smat == SM_SS)
             run_chain<SM_SS, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
-        else if (G.smat == SM_TORCH) {
-            if constexpr (!FAST)
-                run_chain<SM_TORCH, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
-        } else
+        else
             run_chain<SM_IA4, TR, CL, FAST>(A, G, pi, c, xf, out, lout, lists_p, lists_n, list_na, list_cap, L, trace);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -1043,7 +1044,8 @@ struct LdsPlan {
 // W wavefronts per SIMD: W = 3 caps the kernel at 168 VGPRs (three resident
 // wavefronts hide each other's latency; the spills this costs sit outside
 // the fill loop), W = 2 at 256.  Which one runs is a launch-tuning choice.
-template <bool FAST, int W>
+// TORCH (FP64 only) runs the PyTorch engine's DP chains.
+template <bool FAST, int W, bool TORCH>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void chain_kernel(Arena A, const ChainGroup *groups,
                                                    int n_groups, long long n_chains,
                                                    const Hit *xf, Hit *out, ListOut *lout,
@@ -1066,18 +1068,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void ch
     if (trace_g) {
         unsigned *tg = trace_g + (long long)blockIdx.x * slot_words;
         if (cols_in_lds)
-            chain_loop<unsigned, true, FAST>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+            chain_loop<unsigned, true, FAST, TORCH>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
                                        lists_n, list_na, list_cap, queue, L, tg);
         else
-            chain_loop<unsigned, false, FAST>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+            chain_loop<unsigned, false, FAST, TORCH>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
                                         lists_n, list_na, list_cap, queue, L, tg);
     } else {
         typedef __attribute__((address_space(3))) unsigned lds_u32;
         if (cols_in_lds)
-            chain_loop<lds_u32, true, FAST>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+            chain_loop<lds_u32, true, FAST, TORCH>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
                                       lists_n, list_na, list_cap, queue, L, (lds_u32 *)L.trace);
         else
-            chain_loop<lds_u32, false, FAST>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
+            chain_loop<lds_u32, false, FAST, TORCH>(A, groups, n_groups, n_chains, xf, out, lout, lists_p,
                                        lists_n, list_na, list_cap, queue, L, (lds_u32 *)L.trace);
     }
 }
@@ -1158,15 +1160,15 @@ void launch_nw_matrix(hipStream_t s, const NwProblem *probs, int n, const double
 #undef IA_LAUNCH_NWM
 }
 
-int chain_blocks_per_cu(size_t lds, bool fast, int waves) {
+int chain_blocks_per_cu(size_t lds, bool fast, int waves, bool torch) {
     int nb = 0;
     hipError_t e;
+#define IA_OCC(F, W, T) hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<F, W, T>, 64, lds)
     if (waves == 2)
-        e = fast ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<true, 2>, 64, lds)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<false, 2>, 64, lds);
+        e = torch ? IA_OCC(false, 2, true) : fast ? IA_OCC(true, 2, false) : IA_OCC(false, 2, false);
     else
-        e = fast ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<true, 3>, 64, lds)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, chain_kernel<false, 3>, 64, lds);
+        e = torch ? IA_OCC(false, 3, true) : fast ? IA_OCC(true, 3, false) : IA_OCC(false, 3, false);
+#undef IA_OCC
     return e == hipSuccess ? nb : 1;
 }
 
@@ -1190,21 +1192,23 @@ void launch_chains(hipStream_t s, const Arena &A, const ChainGroup *groups, int 
                    int *lists_n, int *list_na, int list_cap, int max_plen, int max_nlen,
                    int *queue, unsigned *trace_scratch, long long trace_slot_words,
                    int n_slots, bool cols_in_lds, bool fast, int waves, P8 *gq,
-                   long long gq_stride) {
+                   long long gq_stride, bool torch) {
     if (n_chains <= 0) return;
     const size_t lds = chain_lds_bytes(max_plen, max_nlen, trace_scratch == nullptr, cols_in_lds);
     const int grid = n_slots;
-#define IA_LAUNCH_CHAIN(F, W)                                                                    \
-    hipLaunchKernelGGL((chain_kernel<F, W>), dim3(grid), dim3(64), lds, s, A, groups, n_groups, \
+#define IA_LAUNCH_CHAIN(F, W, T)                                                                 \
+    hipLaunchKernelGGL((chain_kernel<F, W, T>), dim3(grid), dim3(64), lds, s, A, groups, n_groups, \
                        n_chains, xf, out, lout, lists_p, lists_n, list_na, list_cap, max_plen,   \
                        max_nlen, queue, trace_scratch, trace_slot_words, (int)cols_in_lds, gq,  \
                        gq_stride)
     if (waves == 2) {
-        if (fast) IA_LAUNCH_CHAIN(true, 2);
-        else IA_LAUNCH_CHAIN(false, 2);
+        if (torch) IA_LAUNCH_CHAIN(false, 2, true);
+        else if (fast) IA_LAUNCH_CHAIN(true, 2, false);
+        else IA_LAUNCH_CHAIN(false, 2, false);
     } else {
-        if (fast) IA_LAUNCH_CHAIN(true, 3);
-        else IA_LAUNCH_CHAIN(false, 3);
+        if (torch) IA_LAUNCH_CHAIN(false, 3, true);
+        else if (fast) IA_LAUNCH_CHAIN(true, 3, false);
+        else IA_LAUNCH_CHAIN(false, 3, false);
     }
 #undef IA_LAUNCH_CHAIN
 }
