@@ -157,6 +157,12 @@ int indialign_release(void);
  *       pass 26 n (transform + distance) + 31 n_sel + 3400 (centroids,
  *       covariance, key matrix, Jacobi solve) when n_sel >= 3, the initial
  *       Kabsch 31 n_seed + 3400, and 29 n per TM-score (n the bank's points)
+ *  [11] chain-kernel algorithmic FP64 operations outside the NW cells,
+ *       counted on the device in the same units: per NW iteration the
+ *       Kabsch on the aligned set (31 n + 3400) and its TM-score (29 n);
+ *       per lite detailed search and d0 cutoff, 26 n per selection pass,
+ *       31 n_sel + 3400 per solved Kabsch and 29 n for the TM-score; the
+ *       Kabsch of fragment and offset chain starts (n the set's points)
  * Returns the number of values written. */
 int indialign_last_stats(double *out, int n);
 

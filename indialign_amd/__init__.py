@@ -277,10 +277,10 @@ def set_learned_options(reduce_rescored: bool = False, fallback_top_m: int = 512
 
 def last_stats() -> dict:
     """Work counters of the last call (see indialign_last_stats in the header)."""
-    buf = (ctypes.c_double * 11)()
-    load_library().indialign_last_stats(buf, 11)
+    buf = (ctypes.c_double * 12)()
+    load_library().indialign_last_stats(buf, 12)
     keys = ["wall_ms", "candidates", "nw_cells", "chains", "launches", "chain_ms", "bank_ms",
-            "fallbacks", "fallback_budget_passes", "refine_passes", "bank_ops"]
+            "fallbacks", "fallback_budget_passes", "refine_passes", "bank_ops", "chain_ops"]
     return {k: buf[i] for i, k in enumerate(keys)}
 
 

@@ -95,7 +95,7 @@ constexpr double kNever = 1e300;  // early termination disabled
 
 }  // namespace
 
-constexpr int kNStats = 11;
+constexpr int kNStats = 12;
 constexpr int kTooLong = -6;      // return code: some pair exceeds chain_max_nlen()
 constexpr int kUnsupported = -7;  // return code: method 3 with separate pred/native masks
 constexpr int kChainClasses = 4;  // compact native length <= 160, 400, 900, longer
@@ -488,7 +488,8 @@ void Engine::prepare(const NativePairInput *inp, int n) {
     A_.ssp = b_ssp.get<char>(std::max(off, 1));
     A_.ssn = b_ssn.get<char>(std::max(off, 1));
     A_.info = upload(b_info, info);
-    A_.bank_work = b_bank_work.get<unsigned long long>(2);
+    A_.bank_work = b_bank_work.get<unsigned long long>(3);
+    A_.chain_work = A_.bank_work + 2;
     IA_CHECK(hipStreamSynchronize(s_));
 }
 
@@ -1796,7 +1797,7 @@ int Engine::search(const NativeConfig *cfg, int serial_bank, NativePairResult *o
         if (status_[k] == 2) rc = kTooLong;
     }
     if (!hp_.empty()) {
-        IA_CHECK(hipMemsetAsync(A_.bank_work, 0, 2 * sizeof(unsigned long long), s_));
+        IA_CHECK(hipMemsetAsync(A_.bank_work, 0, 3 * sizeof(unsigned long long), s_));
         enumerate_seeds();
         stage_seed_bank();
         if (torch_sem_) {
@@ -1811,9 +1812,10 @@ int Engine::search(const NativeConfig *cfg, int serial_bank, NativePairResult *o
         stage_dp();
         stage_weighted();
     done:
-        const std::vector<unsigned long long> bw = download(A_.bank_work, 2);
+        const std::vector<unsigned long long> bw = download(A_.bank_work, 3);
         stats[9] = (double)bw[1];
         stats[10] = (double)bw[0];
+        stats[11] = (double)bw[2];
         for (size_t a = 0; a < hp_.size(); a++) {
             NativePairResult &o = out[hp_[a].src];
             o.score = best_[a].score;

@@ -64,6 +64,10 @@ struct Arena {
     // loop and TM-scores (per pass 26 n + 31 n_sel + 3400, score 29 n),
     // [1] refinement passes
     unsigned long long *bank_work;
+    // chain-kernel work counter: algorithmic FP64 operations of the chains'
+    // superpositions and scores outside the NW cells (Kabsch 31 n + 3400,
+    // TM-score 29 n, lite-search passes 26 n), as the seed banks count them
+    unsigned long long *chain_work;
 };
 
 IA_HD void set_identity(double *R, double *t) {

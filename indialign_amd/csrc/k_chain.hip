@@ -43,9 +43,22 @@ struct ChainLds {
     char *cols;                  // compact native secondary structure
     double *rowHG;    // row hand-over buffer: (H, G) of one row's nlen cells
     short *alP, *alN; // aligned compact indices, reverse order (alias rowHG)
-    int *misc;
+    int *misc;        // [0] aligned-pair count, [2..3] the operation count (ops_word)
     P8 *gq;           // this wavefront's global slot: the aligned pairs as 64-byte records
 };
+
+// The wavefront's count of algorithmic FP64 operations outside the NW cells
+// (Arena::chain_work): Kabsch 31 n + 3400 per solve, TM-score 29 n,
+// selection pass 26 n, n the points of the set.  It lives in the
+// wavefront's LDS, so no register holds it across the fill.
+IA_D unsigned long long *ops_word(const ChainLds &L) { return (unsigned long long *)(L.misc + 2); }
+
+// Add v to the count: once per wavefront (v wave-uniform), or once per
+// lite-search group (the lanes with on set).
+IA_D void add_ops(const ChainLds &L, unsigned v, bool on) {
+    if (on) atomicAdd(ops_word(L), (unsigned long long)v);
+}
+IA_D void add_ops(const ChainLds &L, unsigned v) { add_ops(L, v, (threadIdx.x & 63) == 0); }
 
 // Gather the iteration's aligned pairs into the wavefront's slot in list
 // order (record q = compact pred alP[q], native alN[q]), followed by 32 zero
@@ -581,8 +594,10 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
     // the group's current (R, t) was solved from
     unsigned long long rset = 0;
     bool rset_ok = false;
+    const bool lead = gact && s == 0;  // counts the group's operations
     for (int it = 0; it < 8; it++) {
         if (__all(done)) break;
+        add_ops(L, 26u * na, lead && !done);
         double c[6] = {0, 0, 0, 0, 0, 0};
         int cnt = 0;
         if (FAST) {  // one pass: selection + raw second moments
@@ -610,6 +625,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
 #pragma unroll
             for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
             if (!done && cnt >= 3) {
+                add_ops(L, 31u * cnt + 3400u, lead);
                 const double fc = (double)cnt;
                 const double cp[3] = {c[0] / fc, c[1] / fc, c[2] / fc};
                 const double cq[3] = {c[3] / fc, c[4] / fc, c[5] / fc};
@@ -683,6 +699,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
 #pragma unroll
             for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
             if (solve) {
+                add_ops(L, 31u * cnt + 3400u, lead);
                 solve_rotation_quad<FAST>(h, R);
                 finish_t(cp, cq, R, t);
                 rset = selmask;
@@ -712,6 +729,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
     }
     acc = group8_sum(acc);
     const double score = acc / fmax(pi.Lnorm, 1.0);
+    add_ops(L, 29u * na, lead);
     // first maximum over the cutoffs in order (search.h:375)
     double bs = kNeg;
     int bg = -1;
@@ -802,6 +820,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
                         a = A.qcx[off + ns + q]; b = A.qcy[off + ns + q]; cc = A.qcz[off + ns + q];
                     },
                     R, t);
+        add_ops(L, 31u * flen + 3400u);
     } else {
         const int k = G.k0 + item * G.kstep;
         const int N = pi.N;
@@ -843,6 +862,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             for (int q = 0; q < 9; q++) h[q] = wave_sum(h[q]);
             solve_rotation_quad<FAST>(h, R);
             finish_t(cp, cq, R, t);
+            add_ops(L, 31u * cnt + 3400u);
         }
     }
     Hit best;
@@ -891,6 +911,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
                     acc += in ? tm_term(dist2_moved(nR, nt, x, y, z, a, b, cc), d0sq) : 0.0;
                 }
                 sc = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
+                add_ops(L, 60u * nv + 3400u);  // Kabsch + TM-score over the valid entries
             } else {
                 set_identity(nR, nt);
             }
@@ -932,6 +953,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
                                                A.qcx[off + an], A.qcy[off + an], A.qcz[off + an]), d0sq);
                 }
                 sc = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
+                add_ops(L, 60u * na + 3400u);  // Kabsch + TM-score
             } else {
                 set_identity(nR, nt);
             }
@@ -970,6 +992,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             acc += tm_term(dist2_moved(nR, nt, r.p.x, r.p.y, r.p.z, r.q.x, r.q.y, r.q.z), d0sq);
         }
         const double it_score = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
+        add_ops(L, 60u * na + 3400u);  // Kabsch + TM-score
         Hit det;
         lite_search<FAST>(A, pi, L, na, nR, nt, G.sd8, det);
         Hit step;
@@ -1018,6 +1041,8 @@ IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, lon
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         __builtin_amdgcn_wave_barrier();
     }
+    // one atomic per wavefront: its chains' operations outside the NW cells
+    if (lane == 0 && *ops_word(L) > 0) atomicAdd(&A.chain_work[0], *ops_word(L));
 }
 
 // LDS carve-out of one wavefront (shared with chain_lds_bytes).  The
@@ -1065,6 +1090,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void ch
     L.alN = (short *)(lds + P.alN);
     L.trace = (unsigned *)(lds + P.trace);
     L.gq = gq + (long long)blockIdx.x * gq_stride;
+    if (threadIdx.x == 0) *ops_word(L) = 0;
     if (trace_g) {
         unsigned *tg = trace_g + (long long)blockIdx.x * slot_words;
         if (cols_in_lds)

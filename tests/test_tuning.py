@@ -77,7 +77,9 @@ def test_launch_shapes_give_identical_results():
     fast methods: every launch shape (chain kernel variant, slots, trace
     placement, seed-bank pass cap with its compacted continuation launch,
     chain length classes) returns the same scores, rotations and
-    translations."""
+    translations, and the same device-counted algorithmic work (the work
+    depends on the search alone, so a counter lost or counted twice under
+    some shape shows here)."""
     if not ia.gpu_available():
         pytest.fail("no GPU visible to the HIP runtime")
     before = ia.get_launch_tuning()
@@ -85,19 +87,24 @@ def test_launch_shapes_give_identical_results():
     pairs = synth.short_pairs(24) + synth.long_pairs(1) + synth.mixed_pairs(6, first_seed=50101)
     try:
         for method in (0, 1):
-            ref = None
+            ref = ref_work = None
             for shape in [(3, 0, 0, 0), (2, 0, 0, 0), (3, 4, 2, 0), (2, 8, 1, 2), (3, 0, 1, 3),
                           (3, 0, 0, 1), (3, 0, 0, 6), (3, 0, 0, 5, 1), (3, 0, 0, 5, 2),
                           (3, 0, 2, 5, 4)]:
                 ia.set_launch_tuning(*shape[:4])
                 ia.set_chain_buckets(shape[4] if len(shape) > 4 else 1)
                 out = ia.tmscore_search_batch(pairs, method=method)
-                print(f"method {method} shape {shape}: mean score {out[0].mean():.12f}", flush=True)
+                st = ia.last_stats()
+                work = (st["bank_ops"], st["refine_passes"], st["chain_ops"], st["nw_cells"])
+                print(f"method {method} shape {shape}: mean score {out[0].mean():.12f}, "
+                      f"work {work}", flush=True)
+                assert st["chain_ops"] > 0 and st["bank_ops"] > 0
                 if ref is None:
-                    ref = out
+                    ref, ref_work = out, work
                     continue
                 for a, b in zip(out, ref):
                     assert np.array_equal(a, b), (method, shape)
+                assert work == ref_work, (method, shape)
     finally:
         ia.set_launch_tuning(*before)
         ia.set_chain_buckets(1)
