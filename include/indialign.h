@@ -162,7 +162,8 @@ int indialign_release(void);
  *       Kabsch on the aligned set (31 n + 3400) and its TM-score (29 n);
  *       per lite detailed search and d0 cutoff, 26 n per selection pass,
  *       31 n_sel + 3400 per solved Kabsch and 29 n for the TM-score; the
- *       Kabsch of fragment and offset chain starts (n the set's points)
+ *       Kabsch of fragment and offset chain starts (n the set's points);
+ *       FP64 methods (0 and 3) only, 0 for methods 1 and 2
  * Returns the number of values written. */
 int indialign_last_stats(double *out, int n);
 

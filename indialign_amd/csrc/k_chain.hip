@@ -54,11 +54,16 @@ struct ChainLds {
 IA_D unsigned long long *ops_word(const ChainLds &L) { return (unsigned long long *)(L.misc + 2); }
 
 // Add v to the count: once per wavefront (v wave-uniform), or once per
-// lite-search group (the lanes with on set).
+// lite-search group (the lanes with on set).  Only the FP64 kernels count;
+// the fast method's kernels compile without it (their registers are the
+// tighter ones).
+template <bool FAST>
 IA_D void add_ops(const ChainLds &L, unsigned v, bool on) {
-    if (on) atomicAdd(ops_word(L), (unsigned long long)v);
+    if constexpr (!FAST)
+        if (on) atomicAdd(ops_word(L), (unsigned long long)v);
 }
-IA_D void add_ops(const ChainLds &L, unsigned v) { add_ops(L, v, (threadIdx.x & 63) == 0); }
+template <bool FAST>
+IA_D void add_ops(const ChainLds &L, unsigned v) { add_ops<FAST>(L, v, (threadIdx.x & 63) == 0); }
 
 // Gather the iteration's aligned pairs into the wavefront's slot in list
 // order (record q = compact pred alP[q], native alN[q]), followed by 32 zero
@@ -597,7 +602,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
     const bool lead = gact && s == 0;  // counts the group's operations
     for (int it = 0; it < 8; it++) {
         if (__all(done)) break;
-        add_ops(L, 26u * na, lead && !done);
+        add_ops<FAST>(L, 26u * na, lead && !done);
         double c[6] = {0, 0, 0, 0, 0, 0};
         int cnt = 0;
         if (FAST) {  // one pass: selection + raw second moments
@@ -625,7 +630,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
 #pragma unroll
             for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
             if (!done && cnt >= 3) {
-                add_ops(L, 31u * cnt + 3400u, lead);
+                add_ops<FAST>(L, 31u * cnt + 3400u, lead);
                 const double fc = (double)cnt;
                 const double cp[3] = {c[0] / fc, c[1] / fc, c[2] / fc};
                 const double cq[3] = {c[3] / fc, c[4] / fc, c[5] / fc};
@@ -699,7 +704,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
 #pragma unroll
             for (int k = 0; k < 9; k++) h[k] = group8_sum(h[k]);
             if (solve) {
-                add_ops(L, 31u * cnt + 3400u, lead);
+                add_ops<FAST>(L, 31u * cnt + 3400u, lead);
                 solve_rotation_quad<FAST>(h, R);
                 finish_t(cp, cq, R, t);
                 rset = selmask;
@@ -729,7 +734,7 @@ IA_D void lite_search(const Arena &A, const PairInfo &pi, const ChainLds &L, int
     }
     acc = group8_sum(acc);
     const double score = acc / fmax(pi.Lnorm, 1.0);
-    add_ops(L, 29u * na, lead);
+    add_ops<FAST>(L, 29u * na, lead);
     // first maximum over the cutoffs in order (search.h:375)
     double bs = kNeg;
     int bg = -1;
@@ -820,7 +825,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
                         a = A.qcx[off + ns + q]; b = A.qcy[off + ns + q]; cc = A.qcz[off + ns + q];
                     },
                     R, t);
-        add_ops(L, 31u * flen + 3400u);
+        add_ops<FAST>(L, 31u * flen + 3400u);
     } else {
         const int k = G.k0 + item * G.kstep;
         const int N = pi.N;
@@ -862,7 +867,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             for (int q = 0; q < 9; q++) h[q] = wave_sum(h[q]);
             solve_rotation_quad<FAST>(h, R);
             finish_t(cp, cq, R, t);
-            add_ops(L, 31u * cnt + 3400u);
+            add_ops<FAST>(L, 31u * cnt + 3400u);
         }
     }
     Hit best;
@@ -911,7 +916,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
                     acc += in ? tm_term(dist2_moved(nR, nt, x, y, z, a, b, cc), d0sq) : 0.0;
                 }
                 sc = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
-                add_ops(L, 60u * nv + 3400u);  // Kabsch + TM-score over the valid entries
+                add_ops<FAST>(L, 60u * nv + 3400u);  // Kabsch + TM-score over the valid entries
             } else {
                 set_identity(nR, nt);
             }
@@ -953,7 +958,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
                                                A.qcx[off + an], A.qcy[off + an], A.qcz[off + an]), d0sq);
                 }
                 sc = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
-                add_ops(L, 60u * na + 3400u);  // Kabsch + TM-score
+                add_ops<FAST>(L, 60u * na + 3400u);  // Kabsch + TM-score
             } else {
                 set_identity(nR, nt);
             }
@@ -992,7 +997,7 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             acc += tm_term(dist2_moved(nR, nt, r.p.x, r.p.y, r.p.z, r.q.x, r.q.y, r.q.z), d0sq);
         }
         const double it_score = wave_sum(acc) / fmax(pi.Lnorm, 1.0);
-        add_ops(L, 60u * na + 3400u);  // Kabsch + TM-score
+        add_ops<FAST>(L, 60u * na + 3400u);  // Kabsch + TM-score
         Hit det;
         lite_search<FAST>(A, pi, L, na, nR, nt, G.sd8, det);
         Hit step;
@@ -1042,7 +1047,8 @@ IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, lon
         __builtin_amdgcn_wave_barrier();
     }
     // one atomic per wavefront: its chains' operations outside the NW cells
-    if (lane == 0 && *ops_word(L) > 0) atomicAdd(&A.chain_work[0], *ops_word(L));
+    if constexpr (!FAST)
+        if (lane == 0 && *ops_word(L) > 0) atomicAdd(&A.chain_work[0], *ops_word(L));
 }
 
 // LDS carve-out of one wavefront (shared with chain_lds_bytes).  The
@@ -1090,7 +1096,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void ch
     L.alN = (short *)(lds + P.alN);
     L.trace = (unsigned *)(lds + P.trace);
     L.gq = gq + (long long)blockIdx.x * gq_stride;
-    if (threadIdx.x == 0) *ops_word(L) = 0;
+    if (!FAST && threadIdx.x == 0) *ops_word(L) = 0;
     if (trace_g) {
         unsigned *tg = trace_g + (long long)blockIdx.x * slot_words;
         if (cols_in_lds)

@@ -98,7 +98,8 @@ def test_launch_shapes_give_identical_results():
                 work = (st["bank_ops"], st["refine_passes"], st["chain_ops"], st["nw_cells"])
                 print(f"method {method} shape {shape}: mean score {out[0].mean():.12f}, "
                       f"work {work}", flush=True)
-                assert st["chain_ops"] > 0 and st["bank_ops"] > 0
+                # the chain kernel counts its work in the FP64 methods only
+                assert st["bank_ops"] > 0 and (st["chain_ops"] > 0) == (method == 0)
                 if ref is None:
                     ref, ref_work = out, work
                     continue
