@@ -42,10 +42,15 @@ struct ChainLds {
     double *colv;     // the chain's compact native coordinates, x y z per column
     char *cols;                  // compact native secondary structure
     double *rowHG;    // row hand-over buffer: (H, G) of one row's nlen cells
-    short *alP, *alN; // aligned compact indices, reverse order (alias rowHG)
+    unsigned *al;     // aligned pairs, reverse order: pred | native << 16 (alias rowHG)
     int *misc;        // [0] aligned-pair count, [2..3] the operation count (ops_word)
     P8 *gq;           // this wavefront's global slot: the aligned pairs as 64-byte records
 };
+
+// Aligned pair q of the traceback list (indices below 2^16).
+IA_D int al_p(const ChainLds &L, int q) { return (int)(L.al[q] & 0xffffu); }
+IA_D int al_n(const ChainLds &L, int q) { return (int)(L.al[q] >> 16); }
+IA_D unsigned al_pack(int p, int n) { return (unsigned)p | ((unsigned)n << 16); }
 
 // The wavefront's count of algorithmic FP64 operations outside the NW cells
 // (Arena::chain_work): Kabsch 31 n + 3400 per solve, TM-score 29 n,
@@ -66,7 +71,7 @@ template <bool FAST>
 IA_D void add_ops(const ChainLds &L, unsigned v) { add_ops<FAST>(L, v, (threadIdx.x & 63) == 0); }
 
 // Gather the iteration's aligned pairs into the wavefront's slot in list
-// order (record q = compact pred alP[q], native alN[q]), followed by 32 zero
+// order (record q = compact pred al_p(q), native al_n(q)), followed by 32 zero
 // records, so the passes over them read linear addresses; the loads past na
 // of the lite search's batches land on the zeros.
 IA_D void gather_aligned(const Arena &A, int off, const ChainLds &L, int na) {
@@ -74,8 +79,9 @@ IA_D void gather_aligned(const Arena &A, int off, const ChainLds &L, int na) {
     for (int q = lane; q < na + 32; q += 64) {
         P8 r;
         if (q < na) {
-            r.p = A.pc4[off + L.alP[q]];
-            r.q = A.qc4[off + L.alN[q]];
+            const unsigned e = L.al[q];
+            r.p = A.pc4[off + (int)(e & 0xffffu)];
+            r.q = A.qc4[off + (int)(e >> 16)];
         } else {
             r.p = P4{0.0, 0.0, 0.0, 0.0};
             r.q = P4{0.0, 0.0, 0.0, 0.0};
@@ -467,7 +473,7 @@ IA_D int trace_word_index(int a, int b, int P, int &shift) {
 }
 
 // Needleman-Wunsch fill + traceback; returns the aligned count (pairs in
-// alP/alN, reverse order).  nw.h:10-68 semantics.
+// L.al, reverse order).  nw.h:10-68 semantics.
 template <int SMODE, class TR, bool CL, bool FAST>
 IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const double *t,
                   double gap, double cut8, double d0sq_s, ChainLds &L, TR *trace,
@@ -489,8 +495,7 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
                 const unsigned w = trace[trace_word_index(i - 1, j - 1, P, sh)];
                 const unsigned d = (w >> sh) & 3u;
                 if (d == 0) {
-                    L.alP[k] = (short)(i - 1);
-                    L.alN[k] = (short)(j - 1);
+                    L.al[k] = al_pack(i - 1, j - 1);
                     k++;
                     i--;
                     j--;
@@ -506,41 +511,48 @@ IA_D int nw_align(const Arena &A, const PairInfo &pi, const double *R, const dou
         // trace in global memory: a window of four step-words (lane q holds
         // lane q's words of steps 16 (w0 - k) .. 16 (w0 - k) + 15, k < 4) is
         // loaded with four coalesced loads; the wavefront walks it with
-        // wave-uniform (i, j), reading each cell's word by readlane, and
-        // reloads when the path's step leaves the window (steps only
-        // decrease along the path).
-        int i = plen, j = nlen, k = 0;
-        while (i > 0 && j > 0) {
-            const int w0 = (fill_row_start(i - 1, P) + j - 1) >> 4;
+        // wave-uniform cell (a, b) and step s, reading each cell's word from
+        // lane a & 63 by readlane, and reloads when the path's step leaves
+        // the window.  Along the path the step only decreases: a move that
+        // leaves row a for a - 1 takes 1 step, or P - 63 from lane 0 (the
+        // previous round's lane 63), and a move that leaves column b takes
+        // 1 more.  Aligned pair k goes to lane k & 63 of a register; each 64
+        // of them are stored with one write.
+        int a = plen - 1, b = nlen - 1, k = 0;
+        int s = fill_row_start(a, P) + b;
+        unsigned acc = 0;
+        while (a >= 0 && b >= 0) {
+            const int w0 = s >> 4;
             unsigned wv[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) wv[q] = w0 - q >= 0 ? trace[(long long)(w0 - q) * 64 + lane] : 0u;
-            // the path's word index w0 - (s >> 4) never decreases: walk the
-            // cells of word kk, then move on (one readlane per cell)
 #pragma unroll
             for (int kk = 0; kk < 4; kk++) {
                 const int wk = (int)wv[kk];
-                while (i > 0 && j > 0) {
-                    const int s = fill_row_start(i - 1, P) + j - 1;
-                    if (w0 - (s >> 4) != kk) break;
-                    const unsigned w = (unsigned)__builtin_amdgcn_readlane(wk, (i - 1) & 63);
+                const int lo = (w0 - kk) << 4;  // the word's first step
+                while (a >= 0 && b >= 0 && s >= lo) {
+                    const unsigned w = (unsigned)__builtin_amdgcn_readlane(wk, a & 63);
                     const unsigned d = (w >> ((s & 15) * 2)) & 3u;
+                    // a move to row a - 1 from lane 0 takes P - 64 more steps
+                    const int wrap = (a & 63) == 0 ? P - 64 : 0;
                     if (d == 0) {
-                        if (lane == 0) {
-                            L.alP[k] = (short)(i - 1);
-                            L.alN[k] = (short)(j - 1);
-                        }
+                        acc = lane == (k & 63) ? al_pack(a, b) : acc;
                         k++;
-                        i--;
-                        j--;
+                        if ((k & 63) == 0) L.al[k - 64 + lane] = acc;
+                        s -= 2 + wrap;
+                        a--;
+                        b--;
                     } else if (d == 1) {
-                        i--;
+                        s -= 1 + wrap;
+                        a--;
                     } else {
-                        j--;
+                        s -= 1;
+                        b--;
                     }
                 }
             }
         }
+        if (lane < (k & 63)) L.al[(k & ~63) + lane] = acc;
         if (lane == 0) L.misc[0] = k;
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
@@ -897,13 +909,13 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             const int slot = G.list_slot0 + c;
             auto entry = [&](int q, double &x, double &y, double &z, double &a, double &b,
                              double &cc) -> bool {
-                const int ap = L.alP[q], an = L.alN[q];
+                const int ap = al_p(L, q), an = al_n(L, q);
                 x = A.px[off + ap]; y = A.py[off + ap]; z = A.pz[off + ap];
                 a = A.qx[off + an]; b = A.qy[off + an]; cc = A.qz[off + an];
                 return A.v[off + ap] != 0 && A.v[off + an] != 0;
             };
             int nv = 0;
-            for (int q = lane; q < na; q += 64) nv += (A.v[off + L.alP[q]] != 0 && A.v[off + L.alN[q]] != 0) ? 1 : 0;
+            for (int q = lane; q < na; q += 64) nv += (A.v[off + al_p(L, q)] != 0 && A.v[off + al_n(L, q)] != 0) ? 1 : 0;
             for (int m = 32; m > 0; m >>= 1) nv += __shfl_xor(nv, m, 64);
             double nR[9], nt[3];
             double sc = 0.0;
@@ -922,8 +934,8 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             }
             for (int q = lane; q < na; q += 64) {
                 const int rq = na - 1 - q;
-                lists_p[(long long)slot * list_cap + q] = L.alP[rq];
-                lists_n[(long long)slot * list_cap + q] = L.alN[rq];
+                lists_p[(long long)slot * list_cap + q] = al_p(L, rq);
+                lists_n[(long long)slot * list_cap + q] = al_n(L, rq);
             }
             if (lane == 0) {
                 list_na[slot] = na;
@@ -946,14 +958,14 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             if (na >= 3) {
                 wave_kabsch<FAST>(na,
                             [&](int q, double &x, double &y, double &z, double &a, double &b, double &cc) {
-                                const int ap = L.alP[q], an = L.alN[q];
+                                const int ap = al_p(L, q), an = al_n(L, q);
                                 x = A.pcx[off + ap]; y = A.pcy[off + ap]; z = A.pcz[off + ap];
                                 a = A.qcx[off + an]; b = A.qcy[off + an]; cc = A.qcz[off + an];
                             },
                             nR, nt);
                 double acc = 0.0;
                 for (int q = lane; q < na; q += 64) {
-                    const int ap = L.alP[q], an = L.alN[q];
+                    const int ap = al_p(L, q), an = al_n(L, q);
                     acc += tm_term(dist2_moved(nR, nt, A.pcx[off + ap], A.pcy[off + ap], A.pcz[off + ap],
                                                A.qcx[off + an], A.qcy[off + an], A.qcz[off + an]), d0sq);
                 }
@@ -965,8 +977,8 @@ IA_D void run_chain(const Arena &A, const ChainGroup &G, const PairInfo &pi, int
             // aligned pairs in increasing order, full indices
             for (int q = lane; q < na; q += 64) {
                 const int rq = na - 1 - q;
-                lists_p[(long long)slot * list_cap + q] = A.pmap[off + L.alP[rq]];
-                lists_n[(long long)slot * list_cap + q] = A.nmap[off + L.alN[rq]];
+                lists_p[(long long)slot * list_cap + q] = A.pmap[off + al_p(L, rq)];
+                lists_n[(long long)slot * list_cap + q] = A.nmap[off + al_n(L, rq)];
             }
             if (lane == 0) {
                 list_na[slot] = na;
@@ -1055,14 +1067,13 @@ IA_D void chain_loop(const Arena &A, const ChainGroup *groups, int n_groups, lon
 // traceback's aligned-pair lists reuse the row hand-over buffer, which is
 // dead once the fill has finished (and is cleared by the next fill).
 struct LdsPlan {
-    size_t rowHG, misc, alP, alN, colv, cols, trace, total;
+    size_t rowHG, misc, al, colv, cols, trace, total;
     IA_HD LdsPlan(int max_plen, int max_nlen, bool trace_in_lds, bool cols_in_lds) {
         (void)max_plen;
         const size_t n = (size_t)max_nlen, nc = cols_in_lds ? n : 0;
         size_t o = 0;
         rowHG = o; o += 16 * n;
-        alP = rowHG;  // 2 min(plen, nlen) <= 8 nlen bytes
-        alN = rowHG + 8 * n;
+        al = rowHG;  // 4 min(plen, nlen) <= 16 nlen bytes
         colv = o; o += nc ? 24 * (nc + 1) : 0;  // + a copy of column 0 at index nlen
         misc = o; o += 16;
         cols = o; o += (nc + 15) & ~(size_t)15;
@@ -1092,8 +1103,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W))) void ch
     L.colv = (double *)(lds + P.colv);
     L.misc = (int *)(lds + P.misc);
     L.cols = (char *)(lds + P.cols);
-    L.alP = (short *)(lds + P.alP);
-    L.alN = (short *)(lds + P.alN);
+    L.al = (unsigned *)(lds + P.al);
     L.trace = (unsigned *)(lds + P.trace);
     L.gq = gq + (long long)blockIdx.x * gq_stride;
     if (!FAST && threadIdx.x == 0) *ops_word(L) = 0;
@@ -1131,8 +1141,7 @@ __global__ __launch_bounds__(64) void nw_matrix_kernel(const NwProblem *probs, c
     L.colv = (double *)(lds + P.colv);
     L.misc = (int *)(lds + P.misc);
     L.cols = (char *)(lds + P.cols);
-    L.alP = (short *)(lds + P.alP);
-    L.alN = (short *)(lds + P.alN);
+    L.al = (unsigned *)(lds + P.al);
     L.trace = (unsigned *)(lds + P.trace);
     L.gq = nullptr;
     const int lane = threadIdx.x & 63;
@@ -1158,8 +1167,8 @@ __global__ __launch_bounds__(64) void nw_matrix_kernel(const NwProblem *probs, c
         T[pr.h_off + (long long)(a + 1) * (pr.nlen + 1) + b + 1] = (signed char)((w >> sh) & 3u);
     }
     for (int q = lane; q < na; q += 64) {
-        ap[pr.a_off + q] = L.alP[na - 1 - q];
-        an[pr.a_off + q] = L.alN[na - 1 - q];
+        ap[pr.a_off + q] = al_p(L, na - 1 - q);
+        an[pr.a_off + q] = al_n(L, na - 1 - q);
     }
     if (lane == 0) na_out[blockIdx.x] = na;
 }
