@@ -142,13 +142,25 @@ IA_HD bool jacobi_rot_a(double a[4][4], double &c, double &s) {
     const double theta = (a[Q][Q] - a[P][P]) / (2.0 * apq);
 #ifdef __HIP_DEVICE_COMPILE__
     double tn;
-    if (fabs(theta) < 0x1p511) {
-        const double r = recip_in_range(fabs(theta) + sqrt_in_range(1.0 + theta * theta));
+    const double at = fabs(theta);
+    if (at >= 0x1p27 && at < 0x1p511) {
+        // A nearly converged rotation.  theta^2 rounds to at least 2^54, so
+        // 1 + theta^2 rounds back to it, and the correctly rounded square
+        // root of a rounded square is |theta| (radix 2): the denominator is
+        // exactly 2|theta|.  Then |tn| <= 2^-28, 1 + tn^2 rounds to 1 and
+        // c = 1 / sqrt(1) = 1.  The same values as the general path below.
+        const double r = recip_in_range(at + at);
         tn = theta >= 0 ? r : -r;
+        c = 1.0;
     } else {
-        tn = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+        if (at < 0x1p511) {
+            const double r = recip_in_range(at + sqrt_in_range(1.0 + theta * theta));
+            tn = theta >= 0 ? r : -r;
+        } else {
+            tn = (theta >= 0 ? 1.0 : -1.0) / (at + sqrt(1.0 + theta * theta));
+        }
+        c = recip_in_range(sqrt_in_range(1.0 + tn * tn));
     }
-    c = recip_in_range(sqrt_in_range(1.0 + tn * tn));
 #else
     const double tn = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
     c = 1.0 / sqrt(1.0 + tn * tn);
