@@ -787,6 +787,12 @@ long long Engine::run_chains(std::vector<ChainGroup> &groups, const Hit *xf_dev,
         if (!c.in_lds) trace_words_all += c.slots * c.slot_words;
         c.gq_off = gq_all;
         gq_all += c.slots * (c.mn + 32);  // aligned-pair records (na <= nlen) + 32 zeros
+        static const bool prof = std::getenv("INDIALIGN_PROFILE") != nullptr;
+        if (prof)
+            std::fprintf(stderr,
+                         "[indialign] chain launch: groups=%lld chains=%lld max %dx%d lds=%zu "
+                         "trace=%s slots=%lld\n",
+                         c.ng, c.total, c.mp, c.mn, c.lds, c.in_lds ? "lds" : "global", c.slots);
     }
     // every buffer is sized before the first launch: the launches of the
     // classes are in flight together
