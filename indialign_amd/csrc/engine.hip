@@ -597,9 +597,9 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
     // equals, so a launch ends on cheap blocks.  Every block writes only its
     // own candidates' slots, so the order changes no result.
     std::vector<int> bank_order(banks.size());
-    std::vector<double> bank_cost(banks.size());
+    std::vector<long long> bank_cost(banks.size());
     const bool tlist = !banks.empty() && banks[0].pts == PTS_TLIST;
-    long long cand = 0;
+    long long cand = 0, max_cost = 0;
     for (size_t b = 0; b < banks.size(); b++) {
         banks[b].cand_off = cand;
         cand += banks[b].cand_cap;
@@ -608,12 +608,29 @@ Hit *Engine::run_banks(std::vector<BankDesc> &banks, const std::vector<SeedDesc>
         const int n = banks[b].pts == PTS_LIST ? std::min(h.plen, h.nlen) : banks[b].n;
         if ((banks[b].pts == PTS_TLIST) != tlist)
             throw DevError("PyTorch-engine list banks run in a launch of their own");
-        bank_cost[b] = (double)n * banks[b].iters;
+        bank_cost[b] = std::max(0LL, (long long)n * banks[b].iters);
+        max_cost = std::max(max_cost, bank_cost[b]);
         bank_order[b] = (int)b;
     }
-    std::stable_sort(bank_order.begin(), bank_order.end(),
-                     [&](int a, int b) { return bank_cost[a] > bank_cost[b]; });
+    if (max_cost < (1 << 20)) {
+        // stable counting sort, decreasing cost: the order stable_sort gives
+        // (the threading stage's banks number in the hundreds of thousands)
+        std::vector<int> at((size_t)max_cost + 1, 0);
+        for (long long c : bank_cost) at[(size_t)c]++;
+        int p = 0;
+        for (long long c = max_cost; c >= 0; c--) {
+            const int k = at[(size_t)c];
+            at[(size_t)c] = p;
+            p += k;
+        }
+        for (size_t b = 0; b < banks.size(); b++) bank_order[at[(size_t)bank_cost[b]]++] = (int)b;
+    } else {
+        std::stable_sort(bank_order.begin(), bank_order.end(),
+                         [&](int a, int b) { return bank_cost[a] > bank_cost[b]; });
+    }
     std::vector<int> blk_bank, blk_first;
+    blk_bank.reserve(banks.size());
+    blk_first.reserve(banks.size());
     for (int b : bank_order)
         for (int j = 0; j < banks[b].cand_cap; j += 256) {
             blk_bank.push_back(b);
@@ -1429,6 +1446,8 @@ void Engine::stage_rescue() {
             std::vector<int> cnts;
             std::vector<OffsetItem> items;  // every offset window, pair by pair
             std::vector<long long> slot;    // its result slot (3 per offset)
+            items.reserve(P.size() * 160);  // about 2 (plen + nlen) / (min(plen, nlen) / 50) each
+            slot.reserve(P.size() * 160);
             long long base = 0;
             static const double mult[6] = {2.0, 1.5, 1.0, 0.75, 0.5, 0.25};
             for (int a : P) {
@@ -1498,6 +1517,7 @@ void Engine::stage_rescue() {
                 }
             }
             std::vector<BankDesc> banks;
+            banks.reserve(items.size());
             for (size_t k = 0; k < items.size(); k++) {
                 if (!keep[k]) continue;
                 const OffsetItem &it = items[k];
