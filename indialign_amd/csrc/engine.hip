@@ -1518,6 +1518,7 @@ void Engine::stage_rescue() {
             }
             std::vector<BankDesc> banks;
             banks.reserve(items.size());
+            std::vector<int> seeds_of;  // implicit_seed_count(ov, 5) by overlap length
             for (size_t k = 0; k < items.size(); k++) {
                 if (!keep[k]) continue;
                 const OffsetItem &it = items[k];
@@ -1536,7 +1537,9 @@ void Engine::stage_rescue() {
                 b.sd8 = 100.0;
                 b.init = INIT_SEED;
                 b.et = 1.0;
-                b.cand_cap = implicit_seed_count(it.ov, 5) * 6;
+                if ((size_t)it.ov >= seeds_of.size()) seeds_of.resize((size_t)it.ov + 1, -1);
+                if (seeds_of[it.ov] < 0) seeds_of[it.ov] = implicit_seed_count(it.ov, 5);
+                b.cand_cap = seeds_of[it.ov] * 6;
                 b.out_idx = slot[k];
                 banks.push_back(b);
             }
