@@ -4,7 +4,9 @@ indialign_rotation_from_covariance) against an SVD Kabsch in numpy:
 - noisy data: both give the SVD optimum (same objective value, det +1);
 - the closed form (fast method) agrees with the Jacobi solver (plain method)
   wherever it does not decline, and declines only near repeated top
-  eigenvalues (planar / symmetric configurations)."""
+  eigenvalues (planar / symmetric configurations);
+- the rounding identity behind the device Jacobi's direct form of nearly
+  converged rotations."""
 import ctypes
 
 import numpy as np
@@ -115,3 +117,25 @@ def test_degenerate_cases():
     assert rc == 1
     rc, R = _solve(f, np.zeros((3, 3)), 0)
     assert rc == 0 and abs(np.linalg.det(R) - 1) < 1e-12
+
+
+def test_converged_jacobi_rotation_identity():
+    """The device Jacobi takes tn = 1/(2|theta|) and c = 1 directly when
+    2^27 <= |theta| < 2^511.  That is exact only if, in IEEE binary64 with
+    round to nearest, |theta| + sqrt(1 + theta^2) equals 2|theta| and
+    1/sqrt(1 + tn^2) equals 1 there; numpy's sqrt and divide are correctly
+    rounded, as the device sequences are.  Just below 2^27 the first identity
+    fails for some theta, which is why the threshold sits there."""
+    rng = np.random.default_rng(5)
+    th = np.ldexp(rng.uniform(1.0, 2.0, 400_000), rng.integers(27, 511, 400_000))
+    edge = np.array([2.0 ** 27, np.nextafter(2.0 ** 27, np.inf), np.nextafter(2.0 ** 28, 0.0),
+                     np.nextafter(2.0 ** 511, 0.0)])
+    near = np.nextafter(2.0 ** 27, np.inf) * (1.0 + rng.uniform(0.0, 1e-3, 200_000))
+    th = np.concatenate([th, edge, near])
+    assert th.min() >= 2.0 ** 27 and th.max() < 2.0 ** 511
+    assert np.array_equal(th + np.sqrt(1.0 + th * th), th + th)
+    tn = 1.0 / (th + th)
+    assert np.all(1.0 / np.sqrt(1.0 + tn * tn) == 1.0)
+    # below the threshold the shortcut would change some results
+    lo = np.ldexp(rng.uniform(1.0, 2.0, 200_000), rng.integers(20, 27, 200_000))
+    assert not np.array_equal(lo + np.sqrt(1.0 + lo * lo), lo + lo)
